@@ -1,0 +1,242 @@
+/*
+ * mgcfd_op2.h -- C ABI of the MI355X (gfx950) MG-CFD hot path.
+ *
+ * The reference application (euler3d.cpp) reaches its kernels only through
+ * the OP2 API: sets, maps and dats are declared, then every loop is an
+ * op_par_loop that OP2's code generator turns into a call of
+ *     op_par_loop_<kernel>(const char* name, op_set set, op_arg arg0, ...)
+ * This header is that binding surface, implemented by libmgcfd.so with
+ * hand-written HIP kernels and device-resident dats:
+ *   - the OP2 core calls euler3d.cpp makes (decl / arg / fetch / timers),
+ *   - one op_par_loop_<kernel> entry per reference loop site,
+ *   - a few mgcfd_* extensions (flux-variant selection, device-resident
+ *     multigrid run, kernel timing, last error).
+ * All entry points are plain C: pointers, sizes and small structs by value.
+ * Every op_par_loop_* returns 0 on success and a negative code on failure
+ * (argument mismatch, HIP error); mgcfd_last_error() describes the failure.
+ * The C++ wrapper (op_seq.h) turns a failure into OP2's behaviour: message
+ * and exit(1).
+ *
+ * Threading: one host thread drives one library instance; calls are
+ * stream-ordered on the library's own HIP stream and a call returns only
+ * when any host-visible result (global reductions) is valid.
+ */
+#ifndef MGCFD_OP2_H
+#define MGCFD_OP2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- OP2 access codes (the values OP2 uses in op_arg_dat / op_arg_gbl) ---- */
+typedef enum { OP_READ = 0, OP_WRITE = 1, OP_RW = 2, OP_INC = 3, OP_MIN = 4, OP_MAX = 5 } op_access;
+
+typedef struct op_set_core {
+    int         index;       /* declaration order */
+    int         size;        /* element count */
+    const char* name;
+    void*       impl;        /* library-private state (renumbering, plans) */
+} op_set_core;
+typedef op_set_core* op_set;
+
+typedef struct op_map_core {
+    int         index;
+    op_set      from;
+    op_set      to;
+    int         dim;
+    const char* name;
+    void*       impl;
+} op_map_core;
+typedef op_map_core* op_map;
+
+/* 'name' is public and mutable: euler3d.cpp:556-570,736-771 renames dats
+ * before writing them out. */
+typedef struct op_dat_core {
+    int         index;
+    op_set      set;
+    int         dim;
+    int         size;        /* bytes per element as declared */
+    const char* type;        /* "double" or "int" */
+    const char* name;
+    void*       impl;
+} op_dat_core;
+typedef op_dat_core* op_dat;
+
+#define OP_ARG_DAT 0
+#define OP_ARG_GBL 1
+
+typedef struct op_arg {
+    int       argtype;       /* OP_ARG_DAT or OP_ARG_GBL */
+    op_dat    dat;
+    op_map    map;           /* NULL == OP_ID (direct access) */
+    int       idx;           /* map column, -1 when direct */
+    int       dim;
+    int       size;          /* bytes per element */
+    char*     data;          /* host pointer of an op_arg_gbl */
+    const char* type;
+    op_access acc;
+} op_arg;
+
+#define OP_ID ((op_map)0)
+
+/* ---- OP2 core (euler3d.cpp call sites given per function) ---- */
+
+/* op_init(argc, argv, diags): euler3d.cpp:119-127.  Picks HIP device 0
+ * (or MGCFD_DEVICE) and parses "OP_MAPS_BASE_INDEX=<n>" from argv
+ * (euler3d.cpp:104-117).  Returns 0 or a negative code. */
+int  op_init(int argc, char** argv, int diags);
+/* op_exit(): euler3d.cpp:482,546,824.  Frees every set, map and dat. */
+void op_exit(void);
+
+/* Host-array declarations used by the HDF5 loaders (op_decl_*_hdf5,
+ * euler3d.cpp:243-336).  Data is copied in; maps use the base index given
+ * to op_init (default 0). */
+op_set op_decl_set(int size, const char* name);
+op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char* name);
+op_dat op_decl_dat_char(op_set set, int dim, const char* type, int size,
+                        const char* data, const char* name);
+/* op_decl_dat_temp_char: euler3d.cpp:379-409,688 (zero-initialised). */
+op_dat op_decl_dat_temp_char(op_set set, int dim, const char* type, int size,
+                             const char* name);
+/* op_decl_const: euler3d.cpp:232-238, by name as OP2's generated code passes
+ * it: smoothing_coefficient, ff_variable, ff_flux_contribution_momentum_x/y/z,
+ * ff_flux_contribution_density_energy, mesh_name. */
+int  op_decl_const_char(int dim, const char* type, int size, const char* dat,
+                        const char* name);
+
+op_arg op_arg_dat(op_dat dat, int idx, op_map map, int dim, const char* type, op_access acc);
+op_arg op_arg_gbl_char(char* data, int dim, const char* type, int size, op_access acc);
+
+int  op_get_size(op_set set);                  /* euler3d.cpp:537,700,728-729 */
+/* Device -> host copy in the reference's AoS layout and original element
+ * order (the data behind op_fetch_data_hdf5_file, euler3d.cpp:564,740-770). */
+int  op_fetch_data_char(op_dat dat, char* dest);
+/* Host -> device overwrite of a dat (AoS, original order). */
+int  op_upload_data_char(op_dat dat, const char* src);
+/* op_partition / op_renumber (euler3d.cpp:340-377): on one GPU both become
+ * the locality renumbering of every node set (reverse Cuthill-McKee over its
+ * edge maps) and the matching reorder of edge and boundary sets.  Results are
+ * unaffected; op_fetch_data always returns the original order. */
+int  op_partition(const char* lib, const char* method, op_set set, op_map map, op_dat coords);
+int  op_renumber(op_map map);
+void op_timers(double* cpu, double* wall);     /* euler3d.cpp:450,645 */
+int  op_rank(int* rank);
+
+/* ---- op_par_loop entry points: one per reference loop site ----
+ * Argument lists and access modes are exactly those of the reference call
+ * sites; a mismatch returns a negative code. */
+
+/* misc.h:10-16, site euler3d.cpp:414-415 */
+int op_par_loop_initialize_variables_kernel(const char* name, op_set set, op_arg a0);
+/* misc.h:18-24, sites euler3d.cpp:416-420 */
+int op_par_loop_zero_5d_array_kernel(const char* name, op_set set, op_arg a0);
+/* misc.h:34-38, site euler3d.cpp:424-425 */
+int op_par_loop_zero_1d_array_kernel(const char* name, op_set set, op_arg a0);
+/* misc.h:40-76, site euler3d.cpp:426-431 */
+int op_par_loop_calculate_cell_volumes(const char* name, op_set set, op_arg a0, op_arg a1,
+                                       op_arg a2, op_arg a3, op_arg a4);
+/* misc.h:78-84, sites euler3d.cpp:437-440 */
+int op_par_loop_dampen_ewt(const char* name, op_set set, op_arg a0);
+/* copy_double_kernel.h:6-13, site euler3d.cpp:467-469 */
+int op_par_loop_copy_double_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* time_stepping_kernels.h:13-32, site euler3d.cpp:472-475 */
+int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
+/* time_stepping_kernels.h:34-41, site euler3d.cpp:477-479 (gbl OP_MIN) */
+int op_par_loop_get_min_dt_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* time_stepping_kernels.h:43-64, site euler3d.cpp:485-489 */
+int op_par_loop_compute_step_factor_kernel(const char* name, op_set set, op_arg a0, op_arg a1,
+                                           op_arg a2, op_arg a3);
+/* flux.h:41-208, site euler3d.cpp:498-503 -- the hot loop */
+int op_par_loop_compute_flux_edge_kernel(const char* name, op_set set, op_arg a0, op_arg a1,
+                                         op_arg a2, op_arg a3, op_arg a4);
+/* flux.h:14-39, site euler3d.cpp:505-509 */
+int op_par_loop_compute_bnd_node_flux_kernel(const char* name, op_set set, op_arg a0, op_arg a1,
+                                             op_arg a2, op_arg a3);
+/* time_stepping_kernels.h:66-86, site euler3d.cpp:511-516 */
+int op_par_loop_time_step_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2,
+                                 op_arg a3, op_arg a4);
+/* unstructured_stream.h:7-57, site euler3d.cpp:519-524 */
+int op_par_loop_unstructured_stream_kernel(const char* name, op_set set, op_arg a0, op_arg a1,
+                                           op_arg a2, op_arg a3, op_arg a4);
+/* validation.h:27-35, site euler3d.cpp:528-531 */
+int op_par_loop_residual_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
+/* validation.h:37-44, site euler3d.cpp:534-536 (gbl OP_INC) */
+int op_par_loop_calc_rms_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* validation.h:102-115, sites euler3d.cpp:540-542,665-667 (gbl OP_INC) */
+int op_par_loop_count_bad_vals(const char* name, op_set set, op_arg a0, op_arg a1);
+/* mg.h:29-39, site euler3d.cpp:581-583 */
+int op_par_loop_up_pre_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* mg.h:41-52, site euler3d.cpp:585-588 */
+int op_par_loop_up_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
+/* mg.h:54-64, site euler3d.cpp:590-592 */
+int op_par_loop_up_post_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* mg.h:66-88, site euler3d.cpp:626-631 */
+int op_par_loop_down_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2,
+                            op_arg a3, op_arg a4);
+/* validation.h:46-89, site euler3d.cpp:690-693 */
+int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
+/* validation.h:91-100, site euler3d.cpp:696-698 (gbl OP_INC) */
+int op_par_loop_count_non_zeros(const char* name, op_set set, op_arg a0, op_arg a1);
+
+/* ---- mgcfd_* extensions (no OP2 counterpart) ---- */
+
+/* Edge-flux implementation used by op_par_loop_compute_flux_edge_kernel:
+ *   MGCFD_FLUX_GATHER_EXACT  node-centric gather, incidences in edge order,
+ *                            reference operation order, no FMA contraction:
+ *                            bitwise equal to OP2's sequential backend;
+ *   MGCFD_FLUX_GATHER_FAST   same gather, one reciprocal per density and FMA
+ *                            contraction (deterministic, ulp-level drift);
+ *   MGCFD_FLUX_ATOMIC        edge-parallel with FP64 hardware atomics
+ *                            (non-deterministic summation order). */
+#define MGCFD_FLUX_GATHER_EXACT 0
+#define MGCFD_FLUX_GATHER_FAST  1
+#define MGCFD_FLUX_ATOMIC       2
+#define MGCFD_FLUX_NVARIANTS    3
+int mgcfd_set_flux_variant(int variant);
+int mgcfd_get_flux_variant(void);
+
+/* Last error message of this thread ("" if none). */
+const char* mgcfd_last_error(void);
+
+/* Blocks until all queued device work is complete. */
+int mgcfd_sync(void);
+
+/* Device name and compute-unit count of the current device. */
+int mgcfd_device_info(char* name, int name_len, int* n_cu);
+
+/*
+ * Device-resident multigrid (the loop of euler3d.cpp:452-641 without host
+ * round trips).  The caller binds, per level, the sets/maps/dats that play
+ * the reference roles, then runs cycles.  min_dt, the RMS residual and the
+ * abort conditions stay on the device; the host reads them once per run.
+ * Levels are bound in order 0..levels-1.
+ */
+typedef struct mgcfd_level_bind {
+    op_set nodes, edges, bnd_nodes;
+    op_map edge_to_node, bnd_to_node, node_to_mg_node;   /* last is NULL on the coarsest */
+    op_dat variables, old_variables, residuals, fluxes, step_factors, volumes;
+    op_dat coords, edge_weights, bnd_weights, bnd_groups, up_scratch;
+} mgcfd_level_bind;
+
+int mgcfd_mg_bind(int levels, const mgcfd_level_bind* binds);
+/* Runs 'cycles' multigrid cycles.  use_graph != 0 captures one cycle in a
+ * hipGraph and replays it.  rms_hist (host, may be NULL) receives the L0 RMS
+ * per cycle.  *status: 0 ok, 1 negative min_dt (euler3d.cpp:480-484),
+ * 2 NaN/Inf at L0 (euler3d.cpp:544-548); on an abort the state stops
+ * changing at the failing smoothing step.  Returns 0 or a negative code. */
+int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
+
+/* Times 'iters' back-to-back launches of the edge-flux loop on the bound
+ * level with HIP events on the library stream; *ms_per_call receives the
+ * mean.  'dats' (n_copies >= 1) rotate the variables/fluxes/weights among
+ * independent copies so that the working set can exceed the last-level
+ * cache. */
+int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
+                    const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

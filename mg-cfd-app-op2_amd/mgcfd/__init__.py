@@ -1,0 +1,556 @@
+"""Python binding of libmgcfd (the MI355X MG-CFD hot path) over its C ABI.
+
+The library exposes the OP2 API surface that the reference's euler3d.cpp
+uses (sets, maps, dats, ``op_par_loop_<kernel>`` per loop site) with
+hand-written HIP kernels behind it.  This module is a thin ctypes mirror of
+that ABI plus helpers that declare a multigrid mesh the way euler3d.cpp does
+(euler3d.cpp:243-441) and run its loops.
+
+Nothing here computes on the host: every loop goes to the HIP library, and a
+missing library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgcfd.so")
+
+OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = range(6)
+FLUX_GATHER_EXACT, FLUX_GATHER_FAST, FLUX_ATOMIC = 0, 1, 2
+FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2}
+
+
+class OpSetCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("size", C.c_int), ("name", C.c_char_p), ("impl", C.c_void_p)]
+
+
+class OpMapCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("from_", C.POINTER(OpSetCore)), ("to", C.POINTER(OpSetCore)),
+                ("dim", C.c_int), ("name", C.c_char_p), ("impl", C.c_void_p)]
+
+
+class OpDatCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("set", C.POINTER(OpSetCore)), ("dim", C.c_int), ("size", C.c_int),
+                ("type", C.c_char_p), ("name", C.c_char_p), ("impl", C.c_void_p)]
+
+
+OpSet = C.POINTER(OpSetCore)
+OpMap = C.POINTER(OpMapCore)
+OpDat = C.POINTER(OpDatCore)
+
+
+class OpArg(C.Structure):
+    _fields_ = [("argtype", C.c_int), ("dat", OpDat), ("map", OpMap), ("idx", C.c_int), ("dim", C.c_int),
+                ("size", C.c_int), ("data", C.c_void_p), ("type", C.c_char_p), ("acc", C.c_int)]
+
+
+class LevelBind(C.Structure):
+    _fields_ = [("nodes", OpSet), ("edges", OpSet), ("bnd_nodes", OpSet),
+                ("edge_to_node", OpMap), ("bnd_to_node", OpMap), ("node_to_mg_node", OpMap),
+                ("variables", OpDat), ("old_variables", OpDat), ("residuals", OpDat), ("fluxes", OpDat),
+                ("step_factors", OpDat), ("volumes", OpDat), ("coords", OpDat), ("edge_weights", OpDat),
+                ("bnd_weights", OpDat), ("bnd_groups", OpDat), ("up_scratch", OpDat)]
+
+
+class MeshLevelC(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_edges", C.c_int), ("n_bnd", C.c_int),
+                ("coords", C.POINTER(C.c_double)), ("edge_to_node", C.POINTER(C.c_int)),
+                ("edge_weights", C.POINTER(C.c_double)), ("bnd_to_node", C.POINTER(C.c_int)),
+                ("bnd_group", C.POINTER(C.c_int)), ("bnd_weights", C.POINTER(C.c_double)),
+                ("node_to_mg_node", C.POINTER(C.c_int))]
+
+
+class MeshC(C.Structure):
+    _fields_ = [("levels", C.c_int), ("L", MeshLevelC * 8)]
+
+
+class MgcfdError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libmgcfd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MgcfdError(f"libmgcfd.so not built: {LIB_PATH} missing (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    L.mgcfd_last_error.restype = C.c_char_p
+    L.mgcfd_mesh_last_error.restype = C.c_char_p
+    L.op_init.argtypes = [C.c_int, C.c_void_p, C.c_int]
+    L.op_decl_set.restype = OpSet
+    L.op_decl_set.argtypes = [C.c_int, C.c_char_p]
+    L.op_decl_map.restype = OpMap
+    L.op_decl_map.argtypes = [OpSet, OpSet, C.c_int, C.c_void_p, C.c_char_p]
+    L.op_decl_dat_char.restype = OpDat
+    L.op_decl_dat_char.argtypes = [OpSet, C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_char_p]
+    L.op_decl_dat_temp_char.restype = OpDat
+    L.op_decl_dat_temp_char.argtypes = [OpSet, C.c_int, C.c_char_p, C.c_int, C.c_char_p]
+    L.op_decl_const_char.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_char_p]
+    L.op_arg_dat.restype = OpArg
+    L.op_arg_dat.argtypes = [OpDat, C.c_int, OpMap, C.c_int, C.c_char_p, C.c_int]
+    L.op_arg_gbl_char.restype = OpArg
+    L.op_arg_gbl_char.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.c_int]
+    L.op_fetch_data_char.argtypes = [OpDat, C.c_void_p]
+    L.op_upload_data_char.argtypes = [OpDat, C.c_void_p]
+    L.op_partition.argtypes = [C.c_char_p, C.c_char_p, OpSet, OpMap, OpDat]
+    L.op_get_size.argtypes = [OpSet]
+    L.mgcfd_mg_bind.argtypes = [C.c_int, C.POINTER(LevelBind)]
+    L.mgcfd_mg_run.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.mgcfd_time_flux.argtypes = [C.c_int, C.c_int, C.POINTER(OpMap), C.POINTER(OpDat), C.POINTER(OpDat),
+                                  C.POINTER(OpDat), C.c_int, C.POINTER(C.c_double)]
+    L.mgcfd_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.mgcfd_mesh_box.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
+    L.mgcfd_mesh_knn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
+    L.mgcfd_mesh_free.argtypes = [C.POINTER(MeshC)]
+    for name in LOOPS:
+        fn = getattr(L, "op_par_loop_" + name)
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+# op_par_loop entry points and their argument counts (include/mgcfd_op2.h).
+LOOPS = {
+    "initialize_variables_kernel": 1, "zero_5d_array_kernel": 1, "zero_1d_array_kernel": 1,
+    "calculate_cell_volumes": 5, "dampen_ewt": 1, "copy_double_kernel": 2, "calculate_dt_kernel": 3,
+    "get_min_dt_kernel": 2, "compute_step_factor_kernel": 4, "compute_flux_edge_kernel": 5,
+    "compute_bnd_node_flux_kernel": 4, "time_step_kernel": 5, "unstructured_stream_kernel": 5,
+    "residual_kernel": 3, "calc_rms_kernel": 2, "count_bad_vals": 2, "up_pre_kernel": 2, "up_kernel": 3,
+    "up_post_kernel": 2, "down_kernel": 5, "identify_differences": 3, "count_non_zeros": 2,
+}
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise MgcfdError(f"{what}: {lib().mgcfd_last_error().decode()}")
+
+
+def last_error() -> str:
+    return lib().mgcfd_last_error().decode()
+
+
+# --------------------------------------------------------------- meshes --
+
+def _mesh_to_numpy(p) -> list[dict]:
+    m = p.contents
+    out = []
+    for l in range(m.levels):
+        L = m.L[l]
+        n, e, b = L.n_nodes, L.n_edges, L.n_bnd
+
+        def arr(ptr, count, dtype, shape):
+            if count == 0:
+                return np.zeros(shape, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True).reshape(shape)
+
+        lv = dict(
+            coords=arr(L.coords, 3 * n, np.float64, (n, 3)),
+            edge_to_node=arr(L.edge_to_node, 2 * e, np.int32, (e, 2)),
+            edge_weights=arr(L.edge_weights, 3 * e, np.float64, (e, 3)),
+            bnd_to_node=arr(L.bnd_to_node, b, np.int32, (b,)),
+            bnd_group=arr(L.bnd_group, b, np.int32, (b,)),
+            bnd_weights=arr(L.bnd_weights, 3 * b, np.float64, (b, 3)),
+            node_to_mg_node=(arr(L.node_to_mg_node, n, np.int32, (n,)) if bool(L.node_to_mg_node) else None),
+        )
+        out.append(lv)
+    lib().mgcfd_mesh_free(p)
+    return out
+
+
+def mesh_box(n: int, levels: int = 4, seed: int = 1) -> list[dict]:
+    """Seeded jittered n^3 box with 2x-per-axis coarsening (include/mgcfd_mesh.h)."""
+    p = C.POINTER(MeshC)()
+    if lib().mgcfd_mesh_box(n, levels, seed, C.byref(p)) != 0:
+        raise MgcfdError(lib().mgcfd_mesh_last_error().decode())
+    return _mesh_to_numpy(p)
+
+
+def mesh_knn(n_nodes: int, n_edges: int, levels: int = 4, ratios=None, seed: int = 1) -> list[dict]:
+    """Seeded jittered-point nearest-neighbour graph with greedy agglomeration."""
+    p = C.POINTER(MeshC)()
+    rp = None
+    if ratios is not None:
+        ra = (C.c_double * len(ratios))(*ratios)
+        rp = C.cast(ra, C.c_void_p)
+    if lib().mgcfd_mesh_knn(n_nodes, n_edges, levels, rp, seed, C.byref(p)) != 0:
+        raise MgcfdError(lib().mgcfd_mesh_last_error().decode())
+    return _mesh_to_numpy(p)
+
+
+# ---------------------------------------------------------- freestream --
+
+def freestream(mach: float = 1.2, aoa_deg: float = 0.0) -> dict:
+    """Freestream state and flux contributions (euler3d.cpp:156-189)."""
+    aoa = (3.1415926535897931 / 180.0) * aoa_deg
+    rho, p = 1.4, 1.0
+    c = np.sqrt(1.4 * p / rho)
+    speed = mach * c
+    u = np.array([speed * np.cos(aoa), speed * np.sin(aoa), 0.0])
+    m = rho * u
+    E = rho * (0.5 * (speed * speed)) + (p / (1.4 - 1.0))
+    fmx = np.array([u[0] * m[0] + p, u[0] * m[1], u[0] * m[2]])
+    fmy = np.array([fmx[1], u[1] * m[1] + p, u[1] * m[2]])
+    fmz = np.array([fmx[2], fmy[2], u[2] * m[2] + p])
+    dep = E + p
+    fde = np.array([u[0] * dep, u[1] * dep, u[2] * dep])
+    return dict(smoothing_coefficient=float(np.float32(0.2)), ff_variable=np.array([rho, m[0], m[1], m[2], E]),
+                ff_flux_contribution_momentum_x=fmx, ff_flux_contribution_momentum_y=fmy,
+                ff_flux_contribution_momentum_z=fmz, ff_flux_contribution_density_energy=fde)
+
+
+# ------------------------------------------------------------- runtime --
+
+class Runtime:
+    """One library instance: op_init on entry, op_exit on close."""
+
+    def __init__(self, base_index: int = 0):
+        L = lib()
+        args = [b"mgcfd"]
+        if base_index:
+            args.append(f"OP_MAPS_BASE_INDEX={base_index}".encode())
+        argv = (C.c_char_p * len(args))(*args)
+        self._argv = argv
+        _check(L.op_init(len(args), C.cast(argv, C.c_void_p), 0), "op_init")
+        self.L = L
+        self._keep = []
+        self.open = True
+
+    def close(self):
+        if self.open:
+            self.L.op_exit()
+            self.open = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- declarations
+    def decl_set(self, size: int, name: str):
+        s = self.L.op_decl_set(size, name.encode())
+        if not s:
+            raise MgcfdError(last_error())
+        return s
+
+    def decl_map(self, frm, to, dim: int, data: np.ndarray, name: str):
+        a = np.ascontiguousarray(data, dtype=np.int32)
+        m = self.L.op_decl_map(frm, to, dim, a.ctypes.data, name.encode())
+        if not m:
+            raise MgcfdError(last_error())
+        return m
+
+    def decl_dat(self, st, dim: int, data: np.ndarray | None, name: str, typ: str = "double"):
+        if data is None:
+            size = 8 if typ == "double" else 4
+            d = self.L.op_decl_dat_temp_char(st, dim, typ.encode(), size, name.encode())
+        else:
+            dt = np.float64 if typ == "double" else np.int32
+            a = np.ascontiguousarray(data, dtype=dt)
+            d = self.L.op_decl_dat_char(st, dim, typ.encode(), a.itemsize, a.ctypes.data, name.encode())
+        if not d:
+            raise MgcfdError(last_error())
+        return d
+
+    def decl_const(self, name: str, value):
+        if name == "mesh_name":
+            a = np.array([int(value)], dtype=np.int32)
+            _check(self.L.op_decl_const_char(1, b"int", 4, a.ctypes.data, name.encode()), "op_decl_const")
+            return
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(value, dtype=np.float64)))
+        _check(self.L.op_decl_const_char(a.size, b"double", 8, a.ctypes.data, name.encode()), "op_decl_const")
+
+    def decl_consts(self, fs: dict):
+        for k, v in fs.items():
+            self.decl_const(k, v)
+
+    # -- args
+    def arg(self, dat, idx: int = -1, mp=None, dim: int = 1, typ: str = "double", acc: int = OP_READ) -> OpArg:
+        return self.L.op_arg_dat(dat, idx, mp, dim, typ.encode(), acc)
+
+    def gbl(self, arr: np.ndarray, acc: int) -> OpArg:
+        typ = b"double" if arr.dtype == np.float64 else b"int"
+        self._keep.append(arr)
+        return self.L.op_arg_gbl_char(arr.ctypes.data, arr.size, typ, arr.itemsize, acc)
+
+    def par_loop(self, kernel: str, st, *args):
+        fn = getattr(self.L, "op_par_loop_" + kernel)
+        if len(args) != LOOPS[kernel]:
+            raise MgcfdError(f"{kernel}: expected {LOOPS[kernel]} args, got {len(args)}")
+        rc = fn(kernel.encode(), st, *args)
+        self._keep.clear()
+        if rc != 0:
+            raise MgcfdError(f"op_par_loop {kernel}: {last_error()}")
+
+    # -- data
+    def fetch(self, dat) -> np.ndarray:
+        d = dat.contents
+        n = d.set.contents.size
+        typ = d.type.decode()
+        dt = np.float64 if typ == "double" else np.int32
+        out = np.empty((n, d.dim), dtype=dt)
+        _check(self.L.op_fetch_data_char(dat, out.ctypes.data), "op_fetch_data")
+        return out
+
+    def upload(self, dat, data: np.ndarray):
+        d = dat.contents
+        dt = np.float64 if d.type.decode() == "double" else np.int32
+        a = np.ascontiguousarray(data, dtype=dt)
+        if a.size != d.set.contents.size * d.dim:
+            raise MgcfdError("upload: size mismatch")
+        _check(self.L.op_upload_data_char(dat, a.ctypes.data), "op_upload_data")
+
+    def partition(self):
+        _check(self.L.op_partition(b"PARMETIS", b"KWAY", None, None, None), "op_partition")
+
+    def set_flux_variant(self, v: int):
+        _check(self.L.mgcfd_set_flux_variant(v), "mgcfd_set_flux_variant")
+
+    def sync(self):
+        _check(self.L.mgcfd_sync(), "mgcfd_sync")
+
+    def device_info(self):
+        buf = C.create_string_buffer(256)
+        ncu = C.c_int(0)
+        _check(self.L.mgcfd_device_info(buf, 256, C.byref(ncu)), "mgcfd_device_info")
+        return buf.value.decode(), ncu.value
+
+
+@dataclass
+class LevelHandles:
+    """Sets, maps and dats of one multigrid level, named as euler3d.cpp names them."""
+    nodes: object = None
+    edges: object = None
+    bnd_nodes: object = None
+    edge_to_node: object = None
+    bnd_to_node: object = None
+    node_to_mg_node: object = None
+    coords: object = None
+    edge_weights: object = None
+    bnd_weights: object = None
+    bnd_groups: object = None
+    variables: object = None
+    old_variables: object = None
+    residuals: object = None
+    volumes: object = None
+    step_factors: object = None
+    fluxes: object = None
+    dummy_fluxes: object = None
+    up_scratch: object = None
+    sizes: dict = field(default_factory=dict)
+
+
+class Solver:
+    """Declares a multigrid mesh as euler3d.cpp:243-409 does and runs its loops.
+
+    ``mesh`` is a list of per-level dicts (see mesh_box / mesh_knn).
+    """
+
+    def __init__(self, rt: Runtime, mesh: list[dict], mach: float = 1.2, aoa_deg: float = 0.0,
+                 measure_mem_bound: bool = False):
+        self.rt = rt
+        self.nlev = len(mesh)
+        self.fs = freestream(mach, aoa_deg)
+        rt.decl_consts(self.fs)
+        rt.decl_const("mesh_name", 3)
+        self.lv = []
+        for i, m in enumerate(mesh):
+            h = LevelHandles()
+            n, e, b = len(m["coords"]), len(m["edge_to_node"]), len(m["bnd_to_node"])
+            h.sizes = dict(nodes=n, edges=e, bnd=b)
+            h.nodes = rt.decl_set(n, f"op_nodes_L{i}")
+            h.edges = rt.decl_set(e, f"op_edges_L{i}")
+            h.bnd_nodes = rt.decl_set(b, f"op_bnd_nodes_L{i}")
+            h.edge_to_node = rt.decl_map(h.edges, h.nodes, 2, m["edge_to_node"], "edge-->node")
+            h.bnd_to_node = rt.decl_map(h.bnd_nodes, h.nodes, 1, m["bnd_to_node"], "bnd_node-->node")
+            h.bnd_groups = rt.decl_dat(h.bnd_nodes, 1, m["bnd_group"], "bnd_node-->group", "int")
+            h.edge_weights = rt.decl_dat(h.edges, 3, m["edge_weights"], "edge_weights")
+            h.bnd_weights = rt.decl_dat(h.bnd_nodes, 3, m["bnd_weights"], "bnd_node_weights")
+            h.coords = rt.decl_dat(h.nodes, 3, m["coords"], "node_coordinates")
+            self.lv.append(h)
+        for i in range(1, self.nlev):
+            self.lv[i - 1].node_to_mg_node = rt.decl_map(self.lv[i - 1].nodes, self.lv[i].nodes, 1,
+                                                         mesh[i - 1]["node_to_mg_node"], f"op_node-->mg_node_L{i}")
+        for i, h in enumerate(self.lv):
+            h.variables = rt.decl_dat(h.nodes, 5, None, f"p_variables_L{i}")
+            h.old_variables = rt.decl_dat(h.nodes, 5, None, f"p_old_variables_L{i}")
+            h.residuals = rt.decl_dat(h.nodes, 5, None, f"p_residuals_L{i}")
+            h.volumes = rt.decl_dat(h.nodes, 1, None, f"p_volumes_L{i}")
+            h.step_factors = rt.decl_dat(h.nodes, 1, None, f"p_step_factors_L{i}")
+            h.fluxes = rt.decl_dat(h.nodes, 5, None, f"p_fluxes_L{i}")
+            if measure_mem_bound:
+                h.dummy_fluxes = rt.decl_dat(h.nodes, 5, None, f"p_dummy_fluxes_L{i}")
+            if i > 0:
+                h.up_scratch = rt.decl_dat(h.nodes, 1, None, f"p_up_scratch_L{i}", "int")
+
+    # ---- loop sites of euler3d.cpp (one method per op_par_loop)
+    def setup(self):
+        """euler3d.cpp:412-441: initialise variables, zero fluxes, volumes, damp weights."""
+        rt = self.rt
+        for h in self.lv:
+            rt.par_loop("initialize_variables_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=OP_WRITE))
+            rt.par_loop("zero_5d_array_kernel", h.nodes, rt.arg(h.fluxes, -1, None, 5, acc=OP_WRITE))
+            if h.dummy_fluxes:
+                rt.par_loop("zero_5d_array_kernel", h.nodes, rt.arg(h.dummy_fluxes, -1, None, 5, acc=OP_WRITE))
+            rt.par_loop("zero_1d_array_kernel", h.nodes, rt.arg(h.volumes, -1, None, 1, acc=OP_WRITE))
+            rt.par_loop("calculate_cell_volumes", h.edges,
+                        rt.arg(h.coords, 0, h.edge_to_node, 3, acc=OP_READ),
+                        rt.arg(h.coords, 1, h.edge_to_node, 3, acc=OP_READ),
+                        rt.arg(h.edge_weights, -1, None, 3, acc=OP_RW),
+                        rt.arg(h.volumes, 0, h.edge_to_node, 1, acc=OP_INC),
+                        rt.arg(h.volumes, 1, h.edge_to_node, 1, acc=OP_INC))
+        for h in self.lv:
+            rt.par_loop("dampen_ewt", h.edges, rt.arg(h.edge_weights, -1, None, 3, acc=OP_RW))
+            rt.par_loop("dampen_ewt", h.bnd_nodes, rt.arg(h.bnd_weights, -1, None, 3, acc=OP_RW))
+
+    def flux_edge(self, l: int, fluxes=None):
+        rt, h = self.rt, self.lv[l]
+        f = fluxes if fluxes is not None else h.fluxes
+        rt.par_loop("compute_flux_edge_kernel", h.edges,
+                    rt.arg(h.variables, 0, h.edge_to_node, 5, acc=OP_READ),
+                    rt.arg(h.variables, 1, h.edge_to_node, 5, acc=OP_READ),
+                    rt.arg(h.edge_weights, -1, None, 3, acc=OP_READ),
+                    rt.arg(f, 0, h.edge_to_node, 5, acc=OP_INC),
+                    rt.arg(f, 1, h.edge_to_node, 5, acc=OP_INC))
+
+    def flux_bnd(self, l: int):
+        rt, h = self.rt, self.lv[l]
+        rt.par_loop("compute_bnd_node_flux_kernel", h.bnd_nodes,
+                    rt.arg(h.bnd_groups, -1, None, 1, "int", OP_READ),
+                    rt.arg(h.bnd_weights, -1, None, 3, acc=OP_READ),
+                    rt.arg(h.variables, 0, h.bnd_to_node, 5, acc=OP_READ),
+                    rt.arg(h.fluxes, 0, h.bnd_to_node, 5, acc=OP_INC))
+
+    def stream(self, l: int):
+        rt, h = self.rt, self.lv[l]
+        rt.par_loop("unstructured_stream_kernel", h.edges,
+                    rt.arg(h.variables, 0, h.edge_to_node, 5, acc=OP_READ),
+                    rt.arg(h.variables, 1, h.edge_to_node, 5, acc=OP_READ),
+                    rt.arg(h.edge_weights, -1, None, 3, acc=OP_READ),
+                    rt.arg(h.dummy_fluxes, 0, h.edge_to_node, 5, acc=OP_INC),
+                    rt.arg(h.dummy_fluxes, 1, h.edge_to_node, 5, acc=OP_INC))
+
+    def smooth(self, l: int) -> tuple[float, float | None, int]:
+        """One smoothing step, euler3d.cpp:467-550, loop by loop (host syncs at
+        the global reductions as in the reference).  Returns (min_dt, rms, bad)."""
+        rt, h = self.rt, self.lv[l]
+        rt.par_loop("copy_double_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=OP_READ),
+                    rt.arg(h.old_variables, -1, None, 5, acc=OP_WRITE))
+        rt.par_loop("calculate_dt_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=OP_READ),
+                    rt.arg(h.volumes, -1, None, 1, acc=OP_READ), rt.arg(h.step_factors, -1, None, 1, acc=OP_WRITE))
+        min_dt = np.array([np.finfo(np.float64).max])
+        rt.par_loop("get_min_dt_kernel", h.nodes, rt.arg(h.step_factors, -1, None, 1, acc=OP_READ),
+                    rt.gbl(min_dt, OP_MIN))
+        if min_dt[0] < 0.0:
+            return float(min_dt[0]), None, -1
+        rt.par_loop("compute_step_factor_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=OP_READ),
+                    rt.arg(h.volumes, -1, None, 1, acc=OP_READ), rt.gbl(min_dt, OP_READ),
+                    rt.arg(h.step_factors, -1, None, 1, acc=OP_WRITE))
+        for rk in range(3):
+            self.flux_edge(l)
+            self.flux_bnd(l)
+            rkc = np.array([rk], dtype=np.int32)
+            rt.par_loop("time_step_kernel", h.nodes, rt.gbl(rkc, OP_READ),
+                        rt.arg(h.step_factors, -1, None, 1, acc=OP_READ),
+                        rt.arg(h.fluxes, -1, None, 5, acc=OP_RW),
+                        rt.arg(h.old_variables, -1, None, 5, acc=OP_READ),
+                        rt.arg(h.variables, -1, None, 5, acc=OP_WRITE))
+            if h.dummy_fluxes:
+                self.stream(l)
+        rt.par_loop("residual_kernel", h.nodes, rt.arg(h.old_variables, -1, None, 5, acc=OP_READ),
+                    rt.arg(h.variables, -1, None, 5, acc=OP_READ), rt.arg(h.residuals, -1, None, 5, acc=OP_WRITE))
+        rms = None
+        bad = 0
+        if l == 0:
+            r = np.zeros(1)
+            rt.par_loop("calc_rms_kernel", h.nodes, rt.arg(h.residuals, -1, None, 5, acc=OP_READ), rt.gbl(r, OP_INC))
+            rms = float(np.sqrt(r[0] / h.sizes["nodes"]))
+            cnt = np.zeros(1, dtype=np.int32)
+            rt.par_loop("count_bad_vals", h.nodes, rt.arg(h.variables, -1, None, 5, acc=OP_READ), rt.gbl(cnt, OP_INC))
+            bad = int(cnt[0])
+        return float(min_dt[0]), rms, bad
+
+    def restrict(self, l: int):
+        """MG_UP to level l (euler3d.cpp:579-592)."""
+        rt, f, c = self.rt, self.lv[l - 1], self.lv[l]
+        m = f.node_to_mg_node
+        rt.par_loop("up_pre_kernel", f.nodes, rt.arg(c.variables, 0, m, 5, acc=OP_WRITE),
+                    rt.arg(c.up_scratch, 0, m, 1, "int", OP_WRITE))
+        rt.par_loop("up_kernel", f.nodes, rt.arg(f.variables, -1, None, 5, acc=OP_READ),
+                    rt.arg(c.variables, 0, m, 5, acc=OP_INC), rt.arg(c.up_scratch, 0, m, 1, "int", OP_INC))
+        rt.par_loop("up_post_kernel", c.nodes, rt.arg(c.variables, -1, None, 5, acc=OP_RW),
+                    rt.arg(c.up_scratch, -1, None, 1, "int", OP_READ))
+
+    def prolong(self, l: int):
+        """MG_DOWN to level l (euler3d.cpp:601-631, down_kernel branch)."""
+        rt, f, c = self.rt, self.lv[l], self.lv[l + 1]
+        m = f.node_to_mg_node
+        rt.par_loop("down_kernel", f.nodes, rt.arg(f.variables, -1, None, 5, acc=OP_INC),
+                    rt.arg(f.residuals, -1, None, 5, acc=OP_READ), rt.arg(f.coords, -1, None, 3, acc=OP_READ),
+                    rt.arg(c.residuals, 0, m, 5, acc=OP_READ), rt.arg(c.coords, 0, m, 3, acc=OP_READ))
+
+    def run_loops(self, cycles: int):
+        """The multigrid loop of euler3d.cpp:452-641 driven loop by loop.
+        Returns (status, rms_history, min_dt_history)."""
+        level, up, i = 0, True, 0
+        rms_hist, mdt = [], []
+        while i < cycles:
+            min_dt, rms, bad = self.smooth(level)
+            mdt.append(min_dt)
+            if bad < 0:
+                return 1, rms_hist, mdt
+            if level == 0:
+                rms_hist.append(rms)
+                if bad > 0:
+                    return 2, rms_hist, mdt
+            if self.nlev <= 1:
+                i += 1
+            elif up:
+                level += 1
+                self.restrict(level)
+                if level == self.nlev - 1:
+                    up = False
+            else:
+                level -= 1
+                self.prolong(level)
+                if level == 0:
+                    up, i = True, i + 1
+        return 0, rms_hist, mdt
+
+    def bind(self):
+        binds = (LevelBind * self.nlev)()
+        for i, h in enumerate(self.lv):
+            b = binds[i]
+            b.nodes, b.edges, b.bnd_nodes = h.nodes, h.edges, h.bnd_nodes
+            b.edge_to_node, b.bnd_to_node = h.edge_to_node, h.bnd_to_node
+            if h.node_to_mg_node:
+                b.node_to_mg_node = h.node_to_mg_node
+            b.variables, b.old_variables, b.residuals = h.variables, h.old_variables, h.residuals
+            b.fluxes, b.step_factors, b.volumes = h.fluxes, h.step_factors, h.volumes
+            b.coords, b.edge_weights, b.bnd_weights, b.bnd_groups = h.coords, h.edge_weights, h.bnd_weights, h.bnd_groups
+            if h.up_scratch:
+                b.up_scratch = h.up_scratch
+        _check(self.rt.L.mgcfd_mg_bind(self.nlev, binds), "mgcfd_mg_bind")
+
+    def run_device(self, cycles: int, use_graph: bool = True):
+        """Device-resident multigrid (mgcfd_mg_run).  Returns (status, rms_history)."""
+        self.bind()
+        rms = np.zeros(max(cycles, 1))
+        st = C.c_int(0)
+        _check(self.rt.L.mgcfd_mg_run(cycles, 1 if use_graph else 0,
+                                      rms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(st)), "mgcfd_mg_run")
+        return st.value, rms[:cycles]
+
+    def state(self, l: int, what: str = "variables") -> np.ndarray:
+        return self.rt.fetch(getattr(self.lv[l], what))

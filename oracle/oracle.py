@@ -1,0 +1,191 @@
+"""ctypes wrapper of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/libmgcfd_oracle.so (this project's C restatement of the
+reference kernels, oracle/mgcfd_oracle.c) and, where it was built,
+oracle/_ref/libmgcfd_ref.so (the reference's own kernel headers compiled in
+place, oracle/ref_harness.cpp).  Both expose the same functions (orc_* /
+ref_*) on the reference's AoS host layout.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "libmgcfd_oracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmgcfd_ref.so")
+
+
+class Consts(C.Structure):
+    _fields_ = [("smoothing_coefficient", C.c_double), ("ff_variable", C.c_double * 5),
+                ("ff_flux_contribution_momentum_x", C.c_double * 3),
+                ("ff_flux_contribution_momentum_y", C.c_double * 3),
+                ("ff_flux_contribution_momentum_z", C.c_double * 3),
+                ("ff_flux_contribution_density_energy", C.c_double * 3)]
+
+    def as_dict(self) -> dict:
+        return {k: (np.array(getattr(self, k)[:]) if k != "smoothing_coefficient" else self.smoothing_coefficient)
+                for k, _ in self._fields_}
+
+
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+
+
+class Level(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_edges", C.c_int), ("n_bnd", C.c_int),
+                ("edge_to_node", _PI), ("bnd_to_node", _PI), ("bnd_group", _PI), ("node_to_mg_node", _PI),
+                ("coords", _PD), ("edge_weights", _PD), ("bnd_weights", _PD), ("variables", _PD),
+                ("old_variables", _PD), ("residuals", _PD), ("fluxes", _PD), ("step_factors", _PD),
+                ("volumes", _PD), ("up_scratch", _PI)]
+
+
+def _p(a: np.ndarray):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_PD)
+    return a.ctypes.data_as(_PI)
+
+
+class Oracle:
+    """prefix 'orc' = C restatement, 'ref' = reference kernels compiled in place."""
+
+    def __init__(self, which: str = "orc"):
+        path = ORACLE_LIB if which == "orc" else REF_LIB
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.p = which
+        f = self.fn
+        f("freestream").argtypes = [C.c_double, C.c_double, C.POINTER(Consts)]
+        f("get_min_dt").restype = C.c_double
+        f("calc_rms_sum").restype = C.c_double
+        f("count_bad_vals").restype = C.c_int
+        f("count_non_zeros").restype = C.c_int
+        f("mg_run").argtypes = [C.POINTER(Level), C.c_int, C.c_int, C.POINTER(Consts), _PD, _PD, _PI]
+        f("setup").argtypes = [C.POINTER(Level), C.c_int, C.POINTER(Consts)]
+
+    def fn(self, name):
+        return getattr(self.lib, f"{self.p}_{name}")
+
+    @staticmethod
+    def available(which: str) -> bool:
+        return os.path.exists(ORACLE_LIB if which == "orc" else REF_LIB)
+
+    def freestream(self, mach=1.2, aoa=0.0) -> Consts:
+        c = Consts()
+        self.fn("freestream")(C.c_double(mach), C.c_double(aoa), C.byref(c))
+        return c
+
+    # --- per-loop calls (arrays are modified in place, AoS layout)
+    def initialize_variables(self, var, c):
+        self.fn("initialize_variables")(len(var), C.byref(c), _p(var))
+
+    def calculate_cell_volumes(self, e2n, coords, ew, vol):
+        self.fn("calculate_cell_volumes")(len(e2n), _p(e2n), _p(coords), _p(ew), _p(vol))
+
+    def dampen_ewt(self, w):
+        self.fn("dampen_ewt")(len(w), _p(w))
+
+    def calculate_dt(self, var, vol, dt):
+        self.fn("calculate_dt")(len(var), _p(var), _p(vol), _p(dt))
+
+    def get_min_dt(self, dt) -> float:
+        return self.fn("get_min_dt")(len(dt), _p(dt))
+
+    def compute_step_factor(self, var, vol, min_dt, sf):
+        self.fn("compute_step_factor")(len(vol), _p(var), _p(vol), C.c_double(min_dt), _p(sf))
+
+    def compute_flux_edge(self, e2n, var, ew, flux, c):
+        self.fn("compute_flux_edge")(len(e2n), _p(e2n), _p(var), _p(ew), _p(flux), C.byref(c))
+
+    def compute_bnd_node_flux(self, group, b2n, bw, var, flux, c):
+        self.fn("compute_bnd_node_flux")(len(b2n), _p(group), _p(b2n), _p(bw), _p(var), _p(flux), C.byref(c))
+
+    def time_step(self, rk, sf, flux, old, var):
+        self.fn("time_step")(len(var), rk, _p(sf), _p(flux), _p(old), _p(var))
+
+    def unstructured_stream(self, e2n, var, ew, flux):
+        self.fn("unstructured_stream")(len(e2n), _p(e2n), _p(var), _p(ew), _p(flux))
+
+    def copy_double(self, var, old):
+        self.fn("copy_double")(len(var), _p(var), _p(old))
+
+    def residual(self, old, var, res):
+        self.fn("residual")(len(var), _p(old), _p(var), _p(res))
+
+    def calc_rms_sum(self, res) -> float:
+        return self.fn("calc_rms_sum")(len(res), _p(res))
+
+    def count_bad_vals(self, var) -> int:
+        return self.fn("count_bad_vals")(len(var), _p(var))
+
+    def up_pre(self, f2c, var_c, scratch_c):
+        self.fn("up_pre")(len(f2c), _p(f2c), _p(var_c), _p(scratch_c))
+
+    def up(self, f2c, var_f, var_c, scratch_c):
+        self.fn("up")(len(f2c), _p(f2c), _p(var_f), _p(var_c), _p(scratch_c))
+
+    def up_post(self, var_c, scratch_c):
+        self.fn("up_post")(len(var_c), _p(var_c), _p(scratch_c))
+
+    def down(self, f2c, var_f, res_f, xyz_f, res_c, xyz_c):
+        self.fn("down")(len(f2c), _p(f2c), _p(var_f), _p(res_f), _p(xyz_f), _p(res_c), _p(xyz_c))
+
+    def identify_differences(self, test, master, diff):
+        self.fn("identify_differences")(len(test), _p(test), _p(master), _p(diff))
+
+    def count_non_zeros(self, diff) -> int:
+        return self.fn("count_non_zeros")(len(diff), _p(diff))
+
+    # --- whole solve
+    def solve(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, consts: Consts | None = None,
+              after_setup=None):
+        """Setup (euler3d.cpp:412-441) + multigrid loop (:452-641) on copies of
+        the mesh arrays.  after_setup(list of per-level array dicts) may modify
+        the state between the two (failure-path tests).  Returns
+        dict(status, rms, min_dt, steps, levels=[...])."""
+        c = consts if consts is not None else self.freestream(mach, aoa)
+        nl = len(mesh)
+        arrs = []
+        levels = (Level * nl)()
+        for i, m in enumerate(mesh):
+            n, e, b = len(m["coords"]), len(m["edge_to_node"]), len(m["bnd_to_node"])
+            a = dict(
+                e2n=np.ascontiguousarray(m["edge_to_node"], dtype=np.int32),
+                b2n=np.ascontiguousarray(m["bnd_to_node"], dtype=np.int32),
+                bg=np.ascontiguousarray(m["bnd_group"], dtype=np.int32),
+                n2c=(np.ascontiguousarray(m["node_to_mg_node"], dtype=np.int32)
+                     if m.get("node_to_mg_node") is not None else None),
+                xyz=np.ascontiguousarray(m["coords"], dtype=np.float64),
+                ew=np.array(m["edge_weights"], dtype=np.float64, copy=True),
+                bw=np.array(m["bnd_weights"], dtype=np.float64, copy=True),
+                var=np.zeros((n, 5)), old=np.zeros((n, 5)), res=np.zeros((n, 5)), flux=np.zeros((n, 5)),
+                sf=np.zeros(n), vol=np.zeros(n), scr=np.zeros(n, dtype=np.int32))
+            arrs.append(a)
+            L = levels[i]
+            L.n_nodes, L.n_edges, L.n_bnd = n, e, b
+            L.edge_to_node, L.bnd_to_node, L.bnd_group = _p(a["e2n"]), _p(a["b2n"]), _p(a["bg"])
+            L.node_to_mg_node = _p(a["n2c"]) if a["n2c"] is not None else None
+            L.coords, L.edge_weights, L.bnd_weights = _p(a["xyz"]), _p(a["ew"]), _p(a["bw"])
+            L.variables, L.old_variables, L.residuals = _p(a["var"]), _p(a["old"]), _p(a["res"])
+            L.fluxes, L.step_factors, L.volumes, L.up_scratch = _p(a["flux"]), _p(a["sf"]), _p(a["vol"]), _p(a["scr"])
+        self.fn("setup")(levels, nl, C.byref(c))
+        if after_setup is not None:
+            after_setup(arrs)
+        rms = np.zeros(max(cycles, 1))
+        mdt = np.zeros(max(1, cycles * max(1, 2 * nl - 2) + 8))
+        steps = C.c_int(0)
+        st = self.fn("mg_run")(levels, nl, cycles, C.byref(c), _p(rms), _p(mdt), C.byref(steps))
+        out = []
+        for a in arrs:
+            out.append(dict(variables=a["var"], old_variables=a["old"], residuals=a["res"], fluxes=a["flux"],
+                            step_factors=a["sf"], volumes=a["vol"], edge_weights=a["ew"], bnd_weights=a["bw"],
+                            up_scratch=a["scr"]))
+        return dict(status=st, rms=rms[:cycles], min_dt=mdt[:steps.value], steps=steps.value, levels=out)

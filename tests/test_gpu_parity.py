@@ -1,0 +1,356 @@
+"""GPU parity of every hot-path loop against the CPU oracle, through the C ABI.
+
+Bar: the exact kernels (default edge-flux variant, boundary flux, time step,
+restriction, prolongation, setup, reductions' inputs) must be bitwise equal to
+the oracle's OP2-sequential evaluation.  The fast-gather and FP64-atomic flux
+variants, the RMS reduction (tree order) and any quantity through cbrt are
+held to the relative tolerances written next to each assertion.
+
+Relative error of an [N][5] array per variable v:
+    max_i |gpu[i,v] - oracle[i,v]| / max_i |oracle[i,v]|.
+"""
+import numpy as np
+import pytest
+
+import mgcfd
+from conftest import perturbed_state
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STATE_TOL = 1e-12        # state after N cycles, per variable
+FLUX_FAST_TOL = 1e-12    # one flux pass, fast / atomic variants
+
+
+def rel_err(g, o):
+    g = np.asarray(g, dtype=np.float64).reshape(len(o), -1)
+    o = np.asarray(o, dtype=np.float64).reshape(len(o), -1)
+    scale = np.maximum(np.abs(o).max(axis=0), 1e-300)
+    return (np.abs(g - o).max(axis=0) / scale).max()
+
+
+def bitwise(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64 if np.asarray(a).dtype == np.float64 else np.int32),
+                          np.asarray(b).view(np.uint64 if np.asarray(b).dtype == np.float64 else np.int32))
+
+
+@pytest.fixture
+def rt():
+    r = mgcfd.Runtime()
+    yield r
+    r.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("orc")
+
+
+def oracle_setup(o, mesh, c):
+    """Per-level setup arrays exactly as euler3d.cpp:412-441 produce them."""
+    out = []
+    for m in mesh:
+        n = len(m["coords"])
+        ew = np.array(m["edge_weights"], dtype=np.float64)
+        bw = np.array(m["bnd_weights"], dtype=np.float64)
+        vol = np.zeros(n)
+        var = np.zeros((n, 5))
+        o.initialize_variables(var, c)
+        o.calculate_cell_volumes(np.ascontiguousarray(m["edge_to_node"], np.int32),
+                                 np.ascontiguousarray(m["coords"]), ew, vol)
+        out.append(dict(ew=ew, bw=bw, vol=vol, var=var))
+    for a in out:
+        o.dampen_ewt(a["ew"])
+        o.dampen_ewt(a["bw"])
+    return out
+
+
+MESHES = ["box8", "knn_small"]
+
+
+@pytest.mark.parametrize("meshname", MESHES)
+def test_setup_bitwise(rt, orc, meshname, request):
+    mesh = request.getfixturevalue(meshname)
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
+    s.setup()
+    ref = oracle_setup(orc, mesh, c)
+    for l, a in enumerate(ref):
+        assert bitwise(s.state(l, "variables"), a["var"])
+        assert bitwise(s.state(l, "volumes")[:, 0], a["vol"]), f"volumes L{l}"
+        assert bitwise(s.state(l, "edge_weights"), a["ew"]), f"edge weights L{l}"
+        assert bitwise(s.state(l, "bnd_weights"), a["bw"]), f"bnd weights L{l}"
+
+
+@pytest.mark.parametrize("meshname", MESHES)
+@pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic"])
+def test_flux_edge(rt, orc, meshname, variant, request):
+    mesh = request.getfixturevalue(meshname)
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
+    s.setup()
+    ref = oracle_setup(orc, mesh, c)
+    m0 = mesh[0]
+    n = len(m0["coords"])
+    var = perturbed_state(n, c.ff_variable[:], seed=11)
+    flux0 = perturbed_state(n, np.ones(5), seed=12, amp=1e-6) * 1e-9   # INC onto a non-zero flux
+    rt.upload(s.lv[0].variables, var)
+    rt.upload(s.lv[0].fluxes, flux0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.flux_edge(0)
+    g = s.state(0, "fluxes")
+    o = flux0.copy()
+    orc.compute_flux_edge(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o, c)
+    if variant == "gather_exact":
+        assert bitwise(g, o), f"max rel {rel_err(g, o):.3e}"
+    else:
+        assert rel_err(g, o) <= FLUX_FAST_TOL
+
+
+@pytest.mark.parametrize("meshname", MESHES)
+def test_bnd_flux_bitwise(rt, orc, meshname, request):
+    mesh = request.getfixturevalue(meshname)
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
+    s.setup()
+    ref = oracle_setup(orc, mesh, c)
+    m0 = mesh[0]
+    n = len(m0["coords"])
+    var = perturbed_state(n, c.ff_variable[:], seed=21)
+    groups = np.array(m0["bnd_group"], dtype=np.int32)
+    # exercise every branch of flux.h:29-37: g<0 and g<=2 wall, 3..7 far field, >7 none
+    rng = np.random.default_rng(5)
+    groups = rng.choice(np.array([-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9], dtype=np.int32), size=len(groups))
+    rt.upload(s.lv[0].bnd_groups, groups)
+    rt.upload(s.lv[0].variables, var)
+    s.flux_bnd(0)
+    g = s.state(0, "fluxes")
+    o = np.zeros((n, 5))
+    orc.compute_bnd_node_flux(groups, np.ascontiguousarray(m0["bnd_to_node"], np.int32), ref[0]["bw"], var, o, c)
+    assert bitwise(g, o)
+
+
+def test_stream_bitwise(rt, orc, knn_small):
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, knn_small, 0.5, 3.0, measure_mem_bound=True)
+    s.setup()
+    ref = oracle_setup(orc, knn_small, c)
+    m0 = knn_small[0]
+    var = perturbed_state(len(m0["coords"]), c.ff_variable[:], seed=31)
+    rt.upload(s.lv[0].variables, var)
+    s.stream(0)
+    o = np.zeros_like(var)
+    orc.unstructured_stream(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o)
+    assert bitwise(s.state(0, "dummy_fluxes"), o)
+
+
+def test_time_stepping_loops(rt, orc, knn_small):
+    """copy, calculate_dt, get_min_dt, step factor, time_step, residual, rms, bad values."""
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, knn_small, 0.5, 3.0)
+    s.setup()
+    ref = oracle_setup(orc, knn_small, c)
+    n = len(knn_small[0]["coords"])
+    var = perturbed_state(n, c.ff_variable[:], seed=41)
+    flux = perturbed_state(n, np.ones(5), seed=42) * 1e-6
+    rt.upload(s.lv[0].variables, var)
+    rt.upload(s.lv[0].fluxes, flux)
+    h = s.lv[0]
+    O = mgcfd.OP_READ
+    rt.par_loop("copy_double_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=O),
+                rt.arg(h.old_variables, -1, None, 5, acc=mgcfd.OP_WRITE))
+    assert bitwise(s.state(0, "old_variables"), var)
+    rt.par_loop("calculate_dt_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=O),
+                rt.arg(h.volumes, -1, None, 1, acc=O), rt.arg(h.step_factors, -1, None, 1, acc=mgcfd.OP_WRITE))
+    dt_g = s.state(0, "step_factors")[:, 0]
+    dt_o = np.zeros(n)
+    orc.calculate_dt(var, ref[0]["vol"], dt_o)
+    # dt goes through cbrt, which neither the host libm the reference calls
+    # nor the device library rounds correctly; measured spread up to 4 ulp.
+    # Both must also sit within 4 ulp of an extended-precision evaluation.
+    ulps = np.abs(dt_g.view(np.int64) - dt_o.view(np.int64))
+    assert ulps.max() <= 4, f"dt differs by {ulps.max()} ulp"
+    vol = ref[0]["vol"].astype(np.longdouble)
+    rho, m, E = var[:, 0].astype(np.longdouble), var[:, 1:4].astype(np.longdouble), var[:, 4].astype(np.longdouble)
+    u = m / rho[:, None]
+    ssq = (u * u).sum(axis=1)
+    p = np.longdouble(0.4) * (E - np.longdouble(0.5) * rho * ssq)
+    dt_x = (np.longdouble(0.5) * (np.cbrt(vol) / (np.sqrt(ssq) + np.sqrt(np.longdouble(1.4) * p / rho)))).astype(np.float64)
+    for d in (dt_g, dt_o):
+        assert np.abs(d.view(np.int64) - dt_x.view(np.int64)).max() <= 4
+    mn = np.array([np.finfo(np.float64).max])
+    rt.par_loop("get_min_dt_kernel", h.nodes, rt.arg(h.step_factors, -1, None, 1, acc=O), rt.gbl(mn, mgcfd.OP_MIN))
+    assert mn[0] == dt_g.min()
+    rt.par_loop("compute_step_factor_kernel", h.nodes, rt.arg(h.variables, -1, None, 5, acc=O),
+                rt.arg(h.volumes, -1, None, 1, acc=O), rt.gbl(mn, O),
+                rt.arg(h.step_factors, -1, None, 1, acc=mgcfd.OP_WRITE))
+    sf_o = np.zeros(n)
+    orc.compute_step_factor(var, ref[0]["vol"], mn[0], sf_o)
+    assert bitwise(s.state(0, "step_factors")[:, 0], sf_o)
+    for rk in range(3):
+        rt.upload(s.lv[0].fluxes, flux)
+        rkc = np.array([rk], dtype=np.int32)
+        rt.par_loop("time_step_kernel", h.nodes, rt.gbl(rkc, O), rt.arg(h.step_factors, -1, None, 1, acc=O),
+                    rt.arg(h.fluxes, -1, None, 5, acc=mgcfd.OP_RW), rt.arg(h.old_variables, -1, None, 5, acc=O),
+                    rt.arg(h.variables, -1, None, 5, acc=mgcfd.OP_WRITE))
+        vo, fo = np.zeros((n, 5)), flux.copy()
+        orc.time_step(rk, sf_o, fo, var, vo)
+        assert bitwise(s.state(0, "variables"), vo)
+        assert not s.state(0, "fluxes").any()
+    vnew = s.state(0, "variables")
+    rt.par_loop("residual_kernel", h.nodes, rt.arg(h.old_variables, -1, None, 5, acc=O),
+                rt.arg(h.variables, -1, None, 5, acc=O), rt.arg(h.residuals, -1, None, 5, acc=mgcfd.OP_WRITE))
+    res_o = np.zeros((n, 5))
+    orc.residual(var, vnew, res_o)
+    assert bitwise(s.state(0, "residuals"), res_o)
+    r = np.zeros(1)
+    rt.par_loop("calc_rms_kernel", h.nodes, rt.arg(h.residuals, -1, None, 5, acc=O), rt.gbl(r, mgcfd.OP_INC))
+    # tree-order sum vs sequential sum of n*5 positive terms
+    assert abs(r[0] - orc.calc_rms_sum(res_o)) <= 1e-13 * orc.calc_rms_sum(res_o)
+    bad = vnew.copy()
+    bad[7, 2] = np.nan
+    bad[99, 4] = np.inf
+    bad[100, 0] = -np.inf
+    rt.upload(s.lv[0].variables, bad)
+    cnt = np.zeros(1, dtype=np.int32)
+    rt.par_loop("count_bad_vals", h.nodes, rt.arg(h.variables, -1, None, 5, acc=O), rt.gbl(cnt, mgcfd.OP_INC))
+    assert cnt[0] == orc.count_bad_vals(bad) == 3
+
+
+@pytest.mark.parametrize("meshname", MESHES)
+def test_multigrid_transfer_bitwise(rt, orc, meshname, request):
+    mesh = request.getfixturevalue(meshname)
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
+    s.setup()
+    nf, nc = len(mesh[0]["coords"]), len(mesh[1]["coords"])
+    f2c = np.ascontiguousarray(mesh[0]["node_to_mg_node"], np.int32)
+    vf = perturbed_state(nf, c.ff_variable[:], seed=51)
+    vc = perturbed_state(nc, c.ff_variable[:], seed=52)
+    rt.upload(s.lv[0].variables, vf)
+    rt.upload(s.lv[1].variables, vc)
+    s.restrict(1)
+    oc, sc = vc.copy(), np.zeros(nc, dtype=np.int32)
+    orc.up_pre(f2c, oc, sc)
+    orc.up(f2c, vf, oc, sc)
+    orc.up_post(oc, sc)
+    assert bitwise(s.state(1, "variables"), oc)
+    assert bitwise(s.state(1, "up_scratch")[:, 0], sc)
+    rf = perturbed_state(nf, np.ones(5), seed=53) * 1e-5
+    rc = perturbed_state(nc, np.ones(5), seed=54) * 1e-5
+    rt.upload(s.lv[0].residuals, rf)
+    rt.upload(s.lv[1].residuals, rc)
+    s.prolong(0)
+    of = vf.copy()
+    orc.down(f2c, of, rf, np.ascontiguousarray(mesh[0]["coords"]), rc, np.ascontiguousarray(mesh[1]["coords"]))
+    assert bitwise(s.state(0, "variables"), of)
+
+
+def test_identify_differences(rt, orc, box8):
+    s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    s.setup()
+    n = len(box8[0]["coords"])
+    rng = np.random.default_rng(61)
+    master = rng.uniform(-2, 2, size=(n, 5))
+    master[:10] = 0.0
+    test = master * (1 + rng.choice([0.0, 0.5e-7, 2e-7], size=(n, 5)))
+    test[:5] += 1e-19
+    test[5:10] += 1e-18
+    h = s.lv[0]
+    diff = rt.decl_dat(h.nodes, 5, None, "p_var_diff_L0")
+    mst = rt.decl_dat(h.nodes, 5, master, "p_variables_result_L0")
+    rt.upload(h.variables, test)
+    rt.par_loop("identify_differences", h.nodes, rt.arg(h.variables, -1, None, 5),
+                rt.arg(mst, -1, None, 5), rt.arg(diff, -1, None, 5, acc=mgcfd.OP_WRITE))
+    d_o = np.zeros((n, 5))
+    orc.identify_differences(test, master, d_o)
+    assert bitwise(rt.fetch(diff), d_o)
+    cnt = np.zeros(1, dtype=np.int32)
+    rt.par_loop("count_non_zeros", h.nodes, rt.arg(diff, -1, None, 5), rt.gbl(cnt, mgcfd.OP_INC))
+    assert cnt[0] == orc.count_non_zeros(d_o) > 0
+
+
+# ---------------------------------------------------------------- solves --
+
+def solve_gpu(rt, mesh, cycles, mach, aoa, mode, variant="gather_exact"):
+    s = mgcfd.Solver(rt, mesh, mach, aoa)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.setup()
+    if mode == "loops":
+        st, rms, _ = s.run_loops(cycles)
+    else:
+        st, rms = s.run_device(cycles, use_graph=(mode == "graph"))
+    return st, np.asarray(rms), [s.state(l) for l in range(len(mesh))]
+
+
+@pytest.mark.parametrize("meshname", MESHES)
+@pytest.mark.parametrize("mode", ["loops", "device", "graph"])
+def test_solve_parity(rt, orc, meshname, mode, request):
+    mesh = request.getfixturevalue(meshname)
+    cycles = 3
+    ref = orc.solve(mesh, cycles, 0.5, 3.0)
+    st, rms, states = solve_gpu(rt, mesh, cycles, 0.5, 3.0, mode)
+    assert st == 0 and ref["status"] == 0
+    for l in range(len(mesh)):
+        assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
+    assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
+
+
+def test_graph_equals_eager_and_is_deterministic(rt, knn_small):
+    st1, rms1, s1 = solve_gpu(rt, knn_small, 2, 0.5, 3.0, "device")
+    rt.close()
+    rt2 = mgcfd.Runtime()
+    try:
+        st2, rms2, s2 = solve_gpu(rt2, knn_small, 2, 0.5, 3.0, "graph")
+        st3, rms3, s3 = None, None, None
+    finally:
+        rt2.close()
+    for a, b in zip(s1, s2):
+        assert bitwise(a, b)
+    assert bitwise(rms1, rms2)
+
+
+def test_graph_replays_each_match(rt, orc, box8):
+    """Several graph replays with host work between them, each checked."""
+    s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    s.setup()
+    ref = orc.solve(box8, 4, 0.5, 3.0)
+    # one cycle per run call, state carried on the device between calls
+    for k in range(4):
+        st, _ = s.run_device(1, use_graph=True)
+        assert st == 0
+        got = s.state(0)
+        one = orc.solve(box8, k + 1, 0.5, 3.0)
+        assert rel_err(got, one["levels"][0]["variables"]) <= STATE_TOL
+    assert rel_err(s.state(0), ref["levels"][0]["variables"]) <= STATE_TOL
+
+
+def test_atomic_solve_spread(rt, orc, knn_small):
+    ref = orc.solve(knn_small, 3, 0.5, 3.0)
+    st, rms, states = solve_gpu(rt, knn_small, 3, 0.5, 3.0, "graph", variant="atomic")
+    assert st == 0
+    errs = [rel_err(states[l], ref["levels"][l]["variables"]) for l in range(len(knn_small))]
+    print("atomic variant state spread per level:", errs)
+    assert max(errs) <= STATE_TOL
+
+
+def test_bad_value_abort(rt, orc, box8):
+    """NaN injected after setup: abort code 2 after the first L0 step
+    (euler3d.cpp:539-548), same as the oracle, and the state stops there."""
+    def poison(arrs):
+        arrs[0]["var"][3, 1] = np.nan
+    ref = orc.solve(box8, 3, 0.5, 3.0, after_setup=poison)
+    assert ref["status"] == 2 and ref["steps"] == 1
+    s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    s.setup()
+    v = s.state(0)
+    v[3, 1] = np.nan
+    rt.upload(s.lv[0].variables, v)
+    st, rms = s.run_device(3, use_graph=True)
+    assert st == 2
+    g = s.state(0)
+    o = ref["levels"][0]["variables"]
+    ok = np.isnan(g) == np.isnan(o)
+    assert ok.all()
+    m = ~np.isnan(o)
+    assert rel_err(np.where(m, g, 0), np.where(m, o, 0)) <= STATE_TOL
