@@ -136,6 +136,11 @@ def main():
         rt.upload(s.lv[0].variables, perturbed(N_NODES, s.fs["ff_variable"], seed=100 + k))
     rt.sync()
 
+    st = (C.c_int * 5)()
+    mgcfd._check(rt.L.mgcfd_plan_stats(solvers[0].lv[0].edge_to_node, st), "mgcfd_plan_stats")
+    plan = dict(workgroups=st[0], staged_slots=st[1], incidences=st[2], max_degree=st[3], lds_bytes=st[4])
+    log(f"bench: edge-block plan {plan}")
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     orc = Oracle("orc")
@@ -182,7 +187,7 @@ def main():
         "data": "synthetic: seeded jittered-point kNN mesh, freestream M0.5 a3deg +-1% perturbation",
         "config": {"workload": "compute_flux_edge_kernel, level 0 of the 300K-node/930K-edge kNN mesh",
                    "n_nodes": N_NODES, "n_edges": N_EDGES, "variant": best,
-                   "mesh_copies_rotated": args.copies,
+                   "mesh_copies_rotated": args.copies, "plan": plan,
                    "working_set_MB": round(args.copies * nb / 1e6, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

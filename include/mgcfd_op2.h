@@ -116,9 +116,10 @@ int  op_fetch_data_char(op_dat dat, char* dest);
 /* Host -> device overwrite of a dat (AoS, original order). */
 int  op_upload_data_char(op_dat dat, const char* src);
 /* op_partition / op_renumber (euler3d.cpp:340-377): on one GPU both become
- * the locality renumbering of every node set (reverse Cuthill-McKee over its
- * edge maps) and the matching reorder of edge and boundary sets.  Results are
- * unaffected; op_fetch_data always returns the original order. */
+ * the locality renumbering of every node set (recursive graph bisection over
+ * its edge maps into compact parts of 256 nodes, one per flux workgroup) and
+ * the matching reorder of edge and boundary sets.  Results are unaffected;
+ * op_fetch_data always returns the original order. */
 int  op_partition(const char* lib, const char* method, op_set set, op_map map, op_dat coords);
 int  op_renumber(op_map map);
 void op_timers(double* cpu, double* wall);     /* euler3d.cpp:450,645 */
@@ -180,22 +181,39 @@ int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op
 /* validation.h:91-100, site euler3d.cpp:696-698 (gbl OP_INC) */
 int op_par_loop_count_non_zeros(const char* name, op_set set, op_arg a0, op_arg a1);
 
+/* Dispatch by loop name (the second argument of every op_par_loop call in
+ * euler3d.cpp) to the op_par_loop_<name> entry above; nargs must match.
+ * Unknown names return a negative code: there is no host fallback. */
+int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* args);
+
 /* ---- mgcfd_* extensions (no OP2 counterpart) ---- */
 
 /* Edge-flux implementation used by op_par_loop_compute_flux_edge_kernel:
- *   MGCFD_FLUX_GATHER_EXACT  node-centric gather, incidences in edge order,
- *                            reference operation order, no FMA contraction:
- *                            bitwise equal to OP2's sequential backend;
- *   MGCFD_FLUX_GATHER_FAST   same gather, one reciprocal per density and FMA
+ *   MGCFD_FLUX_GATHER_EXACT  blocked gather: one workgroup per mesh part
+ *                            stages the part's and its halo's thermodynamic
+ *                            state in LDS, each node sums its incidences in
+ *                            edge order with the reference's operation order
+ *                            and no FMA contraction: bitwise equal to OP2's
+ *                            sequential backend (default);
+ *   MGCFD_FLUX_GATHER_FAST   same, one reciprocal per density and FMA
  *                            contraction (deterministic, ulp-level drift);
  *   MGCFD_FLUX_ATOMIC        edge-parallel with FP64 hardware atomics
- *                            (non-deterministic summation order). */
+ *                            (non-deterministic summation order);
+ *   MGCFD_FLUX_NODAL_EXACT   node-centric gather without LDS staging, every
+ *                            incidence re-evaluates the neighbour's state
+ *                            (bitwise equal to the sequential backend). */
 #define MGCFD_FLUX_GATHER_EXACT 0
 #define MGCFD_FLUX_GATHER_FAST  1
 #define MGCFD_FLUX_ATOMIC       2
-#define MGCFD_FLUX_NVARIANTS    3
+#define MGCFD_FLUX_NODAL_EXACT  3
+#define MGCFD_FLUX_NVARIANTS    4
 int mgcfd_set_flux_variant(int variant);
 int mgcfd_get_flux_variant(void);
+
+/* Edge-flux plan of an edge->node map: stats[0] workgroups, [1] staged
+ * nodes per workgroup (LDS slots), [2] block-edge entries, [3] max degree,
+ * [4] largest incidence count of one workgroup. */
+int mgcfd_plan_stats(op_map map, int* stats);
 
 /* Last error message of this thread ("" if none). */
 const char* mgcfd_last_error(void);

@@ -1,18 +1,20 @@
 // gfx950 device kernels for the MG-CFD hot path.
 //
 // Data layout in HBM (set up by runtime.hip):
-//   node dats   SoA, component-major: x[v * ld + i], ld = size rounded up to 64
-//   edge map    int2 per edge (internal ids), edges ordered by lower endpoint
-//   weights     packed double4 {w0, w1, w2, |w|} per edge, rebuilt whenever
-//               the weight dat changes (|w| is the reference's ewt, flux.h:48)
-//   inverse map CSR per target node: the incidences that INC into it, in the
-//               reference's element order (OP2 sequential order), each an
-//               int2 {other endpoint, element | column << 31}
+//   dats         the reference's own AoS layout x[i * dim + v] (double or
+//                int), in the runtime's internal element order; one gathered
+//                node is one ~40-byte record, i.e. one or two cache lines
+//   edge map     int2 per edge (internal ids)
+//   weights      packed double4 {w0, w1, w2, |w|} per edge or per
+//                incidence, rebuilt whenever the weight dat changes (|w| is
+//                the reference's ewt, flux.h:48-50)
+//   inverse map  CSR per target node: the incidences that INC into it, in
+//                the reference's element order (OP2 sequential order)
 //
 // Kernels marked "exact" disable FMA contraction and keep the reference's
 // operation order and true divisions, so they are bitwise equal to the
-// sequential CPU evaluation.  "fast" kernels use one reciprocal of the density
-// and FMA contraction.
+// sequential CPU evaluation.  "fast" kernels use one reciprocal of the
+// density and FMA contraction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +27,8 @@ namespace mgcfd {
 constexpr double kGamma = 1.4;
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kPart = 256;       // nodes per flux workgroup (one mesh part)
+constexpr int kRec = 12;         // doubles per staged node record in LDS
 
 struct Consts {
     double sc;          // smoothing_coefficient
@@ -41,60 +45,91 @@ struct Th {
     double fdx, fdy, fdz;                  // energy flux
 };
 
-// Reference order: u = m / rho, |u|^2, p, c = sqrt(gamma p / rho).
-__device__ __forceinline__ void thermo_exact(double rho, double mx, double my, double mz,
-                                             double E, Th& t)
+// Flux tensor from velocity, momentum and pressure (inlined_funcs.h:70-96).
+__device__ __forceinline__ void flux_tensor_exact(Th& t)
 {
 #pragma clang fp contract(off)
-    t.rho = rho; t.mx = mx; t.my = my; t.mz = mz; t.E = E;
-    t.ux = mx / rho;
-    t.uy = my / rho;
-    t.uz = mz / rho;
-    double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
-    t.speed = sqrt(ssq);
-    t.p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
-    t.c = sqrt(kGamma * t.p / rho);
-    t.fxx = t.ux * mx + t.p;
-    t.fxy = t.ux * my;
-    t.fxz = t.ux * mz;
-    t.fyy = t.uy * my + t.p;
-    t.fyz = t.uy * mz;
-    t.fzz = t.uz * mz + t.p;
-    double de_p = E + t.p;
+    t.fxx = t.ux * t.mx + t.p;
+    t.fxy = t.ux * t.my;
+    t.fxz = t.ux * t.mz;
+    t.fyy = t.uy * t.my + t.p;
+    t.fyz = t.uy * t.mz;
+    t.fzz = t.uz * t.mz + t.p;
+    const double de_p = t.E + t.p;
     t.fdx = t.ux * de_p;
     t.fdy = t.uy * de_p;
     t.fdz = t.uz * de_p;
 }
 
-__device__ __forceinline__ void thermo_fast(double rho, double mx, double my, double mz,
-                                            double E, Th& t)
+__device__ __forceinline__ void flux_tensor_fast(Th& t)
 {
-    t.rho = rho; t.mx = mx; t.my = my; t.mz = mz; t.E = E;
-    double inv = 1.0 / rho;
-    t.ux = mx * inv;
-    t.uy = my * inv;
-    t.uz = mz * inv;
-    double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
-    t.speed = sqrt(ssq);
-    t.p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
-    t.c = sqrt(kGamma * t.p * inv);
-    t.fxx = t.ux * mx + t.p;
-    t.fxy = t.ux * my;
-    t.fxz = t.ux * mz;
-    t.fyy = t.uy * my + t.p;
-    t.fyz = t.uy * mz;
-    t.fzz = t.uz * mz + t.p;
-    double de_p = E + t.p;
+    t.fxx = t.ux * t.mx + t.p;
+    t.fxy = t.ux * t.my;
+    t.fxz = t.ux * t.mz;
+    t.fyy = t.uy * t.my + t.p;
+    t.fyz = t.uy * t.mz;
+    t.fzz = t.uz * t.mz + t.p;
+    const double de_p = t.E + t.p;
     t.fdx = t.ux * de_p;
     t.fdy = t.uy * de_p;
     t.fdz = t.uz * de_p;
 }
 
 template <bool EXACT>
-__device__ __forceinline__ void thermo(double rho, double mx, double my, double mz, double E, Th& t)
+__device__ __forceinline__ void flux_tensor(Th& t)
 {
-    if constexpr (EXACT) thermo_exact(rho, mx, my, mz, E, t);
-    else thermo_fast(rho, mx, my, mz, E, t);
+    if constexpr (EXACT) flux_tensor_exact(t);
+    else flux_tensor_fast(t);
+}
+
+// Reference order: u = m / rho, |u|^2, p, c = sqrt(gamma p / rho).
+__device__ __forceinline__ void thermo_exact(const double* U, Th& t)
+{
+#pragma clang fp contract(off)
+    t.rho = U[0]; t.mx = U[1]; t.my = U[2]; t.mz = U[3]; t.E = U[4];
+    t.ux = t.mx / t.rho;
+    t.uy = t.my / t.rho;
+    t.uz = t.mz / t.rho;
+    const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
+    t.speed = sqrt(ssq);
+    t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
+    t.c = sqrt(kGamma * t.p / t.rho);
+    flux_tensor_exact(t);
+}
+
+__device__ __forceinline__ void thermo_fast(const double* U, Th& t)
+{
+    t.rho = U[0]; t.mx = U[1]; t.my = U[2]; t.mz = U[3]; t.E = U[4];
+    const double inv = 1.0 / t.rho;
+    t.ux = t.mx * inv;
+    t.uy = t.my * inv;
+    t.uz = t.mz * inv;
+    const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
+    t.speed = sqrt(ssq);
+    t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
+    t.c = sqrt(kGamma * t.p * inv);
+    flux_tensor_fast(t);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void thermo(const double* U, Th& t)
+{
+    if constexpr (EXACT) thermo_exact(U, t);
+    else thermo_fast(U, t);
+}
+
+__device__ __forceinline__ void load5(const double* __restrict__ x, int i, double U[5])
+{
+    const double* p = x + (size_t)i * 5;
+#pragma unroll
+    for (int v = 0; v < 5; v++) U[v] = p[v];
+}
+
+__device__ __forceinline__ void store5(double* __restrict__ x, int i, const double U[5])
+{
+    double* p = x + (size_t)i * 5;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p[v] = U[v];
 }
 
 // Increment that edge (own, oth) adds to node 'own' (flux.h:139-207).  The
@@ -105,7 +140,7 @@ __device__ __forceinline__ void edge_increment_exact(const Th& o, const Th& t, d
                                                      double fx, double fy, double fz, double inc[5])
 {
 #pragma clang fp contract(off)
-    double factor = -ewt * sc * 0.5 * (o.speed + t.speed + o.c + t.c);
+    const double factor = -ewt * sc * 0.5 * (o.speed + t.speed + o.c + t.c);
     inc[0] = factor * (o.rho - t.rho) + fx * (o.mx + t.mx) + fy * (o.my + t.my) + fz * (o.mz + t.mz);
     inc[4] = factor * (o.E - t.E) + fx * (o.fdx + t.fdx) + fy * (o.fdy + t.fdy) + fz * (o.fdz + t.fdz);
     inc[1] = factor * (o.mx - t.mx) + fx * (o.fxx + t.fxx) + fy * (o.fxy + t.fxy) + fz * (o.fxz + t.fxz);
@@ -116,12 +151,20 @@ __device__ __forceinline__ void edge_increment_exact(const Th& o, const Th& t, d
 __device__ __forceinline__ void edge_increment_fast(const Th& o, const Th& t, double ewt, double sc,
                                                     double fx, double fy, double fz, double inc[5])
 {
-    double factor = -ewt * sc * 0.5 * (o.speed + t.speed + o.c + t.c);
+    const double factor = -ewt * sc * 0.5 * (o.speed + t.speed + o.c + t.c);
     inc[0] = factor * (o.rho - t.rho) + fx * (o.mx + t.mx) + fy * (o.my + t.my) + fz * (o.mz + t.mz);
     inc[4] = factor * (o.E - t.E) + fx * (o.fdx + t.fdx) + fy * (o.fdy + t.fdy) + fz * (o.fdz + t.fdz);
     inc[1] = factor * (o.mx - t.mx) + fx * (o.fxx + t.fxx) + fy * (o.fxy + t.fxy) + fz * (o.fxz + t.fxz);
     inc[2] = factor * (o.my - t.my) + fx * (o.fxy + t.fxy) + fy * (o.fyy + t.fyy) + fz * (o.fyz + t.fyz);
     inc[3] = factor * (o.mz - t.mz) + fx * (o.fxz + t.fxz) + fy * (o.fyz + t.fyz) + fz * (o.fzz + t.fzz);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void edge_increment(const Th& o, const Th& t, double ewt, double sc, double fx,
+                                               double fy, double fz, double inc[5])
+{
+    if constexpr (EXACT) edge_increment_exact(o, t, ewt, sc, fx, fy, fz, inc);
+    else edge_increment_fast(o, t, ewt, sc, fx, fy, fz, inc);
 }
 
 __device__ __forceinline__ double add_exact(double a, double b)
@@ -144,74 +187,176 @@ __device__ __forceinline__ int xcd_block()
 __device__ __forceinline__ bool halted(const int* status) { return status && *status != 0; }
 
 // ------------------------------------------------------------------------
-// Edge flux, node-centric gather.  One thread per target node; the thread
-// evaluates every incident edge in the reference's edge order and adds the
-// increments into a register accumulator seeded with the node's flux, which
-// reproduces OP2's sequential INC order.  No atomics, one launch.
+// Edge flux, blocked gather (default).  Workgroup b owns the nodes
+// [bnode.x, bnode.x + bnode.y) of one compact mesh part; inside a part the
+// nodes are ordered by decreasing degree.
+//  1. Stage: thread t evaluates the thermodynamic state of owned node t and
+//     of halo node t (neighbours owned by other parts, at most kPart per
+//     block) and stores both as kRec-double records in LDS.
+//  2. Walk: thread t adds, in the reference's edge order, the increment of
+//     each of its node's edges, reading the other endpoint from LDS.  The
+//     k-th incidences of all threads of a block are stored contiguously
+//     (jagged-diagonal order: jbase + jpref[k] + t), so the per-incidence
+//     loads {LDS slot | b-role << 31} and {w0, w1, w2, |w|} are coalesced.
+// Every node sums its edges in OP2's sequential order; with EXACT (no FMA
+// contraction, true divisions) the result is bitwise the reference's.
+// Deterministic, no atomics, one launch, one barrier.
+// ------------------------------------------------------------------------
+template <bool EXACT>
+__device__ __forceinline__ void stage_record(double* rec, const double U[5])
+{
+    Th th;
+    thermo<EXACT>(U, th);
+    double2* r = reinterpret_cast<double2*>(rec);
+    r[0] = make_double2(th.rho, th.mx);
+    r[1] = make_double2(th.my, th.mz);
+    r[2] = make_double2(th.E, th.ux);
+    r[3] = make_double2(th.uy, th.uz);
+    r[4] = make_double2(th.p, th.speed);
+    r[5] = make_double2(th.c, 0.0);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ void load_record(const double* rec, Th& t)
+{
+    const double2* r = reinterpret_cast<const double2*>(rec);
+    const double2 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4], f = r[5];
+    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x; t.ux = c.y;
+    t.uy = d.x; t.uz = d.y; t.p = e.x; t.speed = e.y; t.c = f.x;
+    flux_tensor<EXACT>(t);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(kBlock) void k_flux_blocked(
+    const int2* __restrict__ bnode, const int* __restrict__ hptr, const int* __restrict__ halo,
+    const int* __restrict__ jbase, const int* __restrict__ pbase, const int* __restrict__ jpref,
+    const int* __restrict__ nptr, const unsigned* __restrict__ jslot, const double4* __restrict__ jw,
+    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
+{
+    extern __shared__ double sm[];
+    if (halted(status)) return;
+    const int b = xcd_block(), t = threadIdx.x;
+    const int h0 = hptr[b], nh = hptr[b + 1] - h0;
+    const int hn = (t < nh) ? halo[h0 + t] : -1;
+    const int2 bn = bnode[b];
+    const bool own = t < bn.y;
+    const int nd = bn.x + t;
+    double uo[5], uh[5], acc[5];
+    int deg = 0;
+    if (own) {
+        load5(var, nd, uo);
+        load5(flux, nd, acc);
+        deg = nptr[nd + 1] - nptr[nd];
+    }
+    if (hn >= 0) load5(var, hn, uh);
+    if (own) stage_record<EXACT>(sm + t * kRec, uo);
+    if (hn >= 0) stage_record<EXACT>(sm + (kPart + t) * kRec, uh);
+    __syncthreads();
+    if (!own) return;
+    Th o;
+    load_record<EXACT>(sm + t * kRec, o);
+    const int jb = jbase[b] + t, pb = pbase[b];
+    for (int q = 0; q < deg; q++) {
+        const int at = jb + jpref[pb + q];
+        const unsigned r = jslot[at];
+        const double4 w = jw[at];
+        Th th;
+        load_record<EXACT>(sm + (int)(r & 0x7fffffffu) * kRec, th);
+        double fx = -0.5 * w.x, fy = -0.5 * w.y, fz = -0.5 * w.z;
+        if (r & 0x80000000u) { fx = -fx; fy = -fy; fz = -fz; }   // node is endpoint b
+        double d[5];
+        edge_increment<EXACT>(o, th, w.w, k.sc, fx, fy, fz, d);
+        if constexpr (EXACT) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] = add_exact(acc[v], d[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] += d[v];
+        }
+    }
+    store5(flux, nd, acc);
+}
+
+// Per-incidence weights of the blocked gather, gathered from the weight dat.
+__global__ void k_pack_incidence_weights(int n, const int* __restrict__ eid, const double* __restrict__ w,
+                                         double4* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int e = eid[k];
+    if (e < 0) { out[k] = make_double4(0.0, 0.0, 0.0, 0.0); return; }
+    const double a = w[3 * (size_t)e], b = w[3 * (size_t)e + 1], c = w[3 * (size_t)e + 2];
+    out[k] = make_double4(a, b, c, sqrt(a * a + b * b + c * c));
+}
+
+// ------------------------------------------------------------------------
+// Edge flux, node-centric gather without LDS: one thread per node, every
+// incidence re-evaluates the neighbour's state from global memory.
 // ------------------------------------------------------------------------
 template <bool EXACT>
 __global__ __launch_bounds__(kBlock) void k_flux_gather(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int2* __restrict__ inc, const double4* __restrict__ wpk,
-    const double* __restrict__ var, double* __restrict__ flux, int ld, Consts k,
+    const double* __restrict__ var, double* __restrict__ flux, Consts k,
     const int* __restrict__ status)
 {
     if (halted(status)) return;
     const int tid = xcd_block() * kBlock + threadIdx.x;
     if (tid >= n_targets) return;
     const int n = targets ? targets[tid] : tid;
+    double U[5], acc[5];
+    load5(var, n, U);
+    load5(flux, n, acc);
     Th o;
-    thermo<EXACT>(var[n], var[ld + n], var[2 * ld + n], var[3 * ld + n], var[4 * ld + n], o);
-    double acc[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) acc[v] = flux[v * ld + n];
+    thermo<EXACT>(U, o);
     const int beg = ptr[tid], end = ptr[tid + 1];
     for (int q = beg; q < end; q++) {
         const int2 r = inc[q];
-        const int j = r.x;
-        const int e = r.y & 0x7fffffff;
-        const double4 w = wpk[e];
+        const double4 w = wpk[r.y & 0x7fffffff];
+        double X[5];
+        load5(var, r.x, X);
         Th t;
-        thermo<EXACT>(var[j], var[ld + j], var[2 * ld + j], var[3 * ld + j], var[4 * ld + j], t);
+        thermo<EXACT>(X, t);
         double fx = -0.5 * w.x, fy = -0.5 * w.y, fz = -0.5 * w.z;
         if (r.y < 0) { fx = -fx; fy = -fy; fz = -fz; }   // node is endpoint b
         double d[5];
+        edge_increment<EXACT>(o, t, w.w, k.sc, fx, fy, fz, d);
         if constexpr (EXACT) {
-            edge_increment_exact(o, t, w.w, k.sc, fx, fy, fz, d);
 #pragma unroll
             for (int v = 0; v < 5; v++) acc[v] = add_exact(acc[v], d[v]);
         } else {
-            edge_increment_fast(o, t, w.w, k.sc, fx, fy, fz, d);
 #pragma unroll
             for (int v = 0; v < 5; v++) acc[v] += d[v];
         }
     }
-#pragma unroll
-    for (int v = 0; v < 5; v++) flux[v * ld + n] = acc[v];
+    store5(flux, n, acc);
 }
 
 // Edge flux, edge-parallel with FP64 hardware atomics (global_atomic_add_f64).
 __global__ __launch_bounds__(kBlock) void k_flux_atomic(
     int n_edges, const int2* __restrict__ e2n, const double4* __restrict__ wpk,
-    const double* __restrict__ var, double* __restrict__ flux, int ld, Consts k,
-    const int* __restrict__ status)
+    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
 {
     if (halted(status)) return;
     const int e = xcd_block() * kBlock + threadIdx.x;
     if (e >= n_edges) return;
     const int2 ab = e2n[e];
     const double4 w = wpk[e];
+    double Ua[5], Ub[5];
+    load5(var, ab.x, Ua);
+    load5(var, ab.y, Ub);
     Th a, b;
-    thermo_fast(var[ab.x], var[ld + ab.x], var[2 * ld + ab.x], var[3 * ld + ab.x], var[4 * ld + ab.x], a);
-    thermo_fast(var[ab.y], var[ld + ab.y], var[2 * ld + ab.y], var[3 * ld + ab.y], var[4 * ld + ab.y], b);
+    thermo_fast(Ua, a);
+    thermo_fast(Ub, b);
     const double fx = -0.5 * w.x, fy = -0.5 * w.y, fz = -0.5 * w.z;
     double da[5], db[5];
     edge_increment_fast(a, b, w.w, k.sc, fx, fy, fz, da);
     edge_increment_fast(b, a, w.w, k.sc, -fx, -fy, -fz, db);
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        unsafeAtomicAdd(&flux[v * ld + ab.x], da[v]);
-        unsafeAtomicAdd(&flux[v * ld + ab.y], db[v]);
+        unsafeAtomicAdd(&flux[5 * (size_t)ab.x + v], da[v]);
+        unsafeAtomicAdd(&flux[5 * (size_t)ab.y + v], db[v]);
     }
 }
 
@@ -220,46 +365,40 @@ __global__ __launch_bounds__(kBlock) void k_flux_atomic(
 __global__ __launch_bounds__(kBlock) void k_stream_gather(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int2* __restrict__ inc, const double4* __restrict__ wpk,
-    const double* __restrict__ var, double* __restrict__ flux, int ld)
+    const double* __restrict__ var, double* __restrict__ flux)
 {
 #pragma clang fp contract(off)
     const int tid = xcd_block() * kBlock + threadIdx.x;
     if (tid >= n_targets) return;
     const int n = targets ? targets[tid] : tid;
     double acc[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) acc[v] = flux[v * ld + n];
+    load5(flux, n, acc);
     for (int q = ptr[tid]; q < ptr[tid + 1]; q++) {
         const int2 r = inc[q];
-        const int j = r.x;
-        const double u0 = var[j], u1 = var[ld + j], u2 = var[2 * ld + j], u3 = var[3 * ld + j],
-                     u4 = var[4 * ld + j];
+        double u[5];
+        load5(var, r.x, u);
         if (r.y >= 0) {     // node is a: fluxes_a += b-values (+ weights)
             const double4 w = wpk[r.y];
-            acc[0] += u0 + w.x;
-            acc[1] += u1 + w.z;
-            acc[2] += u2;
-            acc[3] += u3;
-            acc[4] += u4 + w.y;
+            acc[0] += u[0] + w.x;
+            acc[1] += u[1] + w.z;
+            acc[2] += u[2];
+            acc[3] += u[3];
+            acc[4] += u[4] + w.y;
         } else {            // node is b: fluxes_b += a-values
-            acc[0] += u0;
-            acc[1] += u1;
-            acc[2] += u2;
-            acc[3] += u3;
-            acc[4] += u4;
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] += u[v];
         }
     }
-#pragma unroll
-    for (int v = 0; v < 5; v++) flux[v * ld + n] = acc[v];
+    store5(flux, n, acc);
 }
 
-// Packed weights {w0, w1, w2, |w|}: |w| = sqrt(w0^2 + w1^2 + w2^2) (flux.h:48-50).
-__global__ void k_pack_weights(int n, const double* __restrict__ w, int ld, double4* __restrict__ out)
+// Packed edge weights {w0, w1, w2, |w|} (flux.h:48-50).
+__global__ void k_pack_weights(int n, const double* __restrict__ w, double4* __restrict__ out)
 {
 #pragma clang fp contract(off)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    const double a = w[e], b = w[ld + e], c = w[2 * ld + e];
+    const double a = w[3 * (size_t)e], b = w[3 * (size_t)e + 1], c = w[3 * (size_t)e + 2];
     out[e] = make_double4(a, b, c, sqrt(a * a + b * b + c * c));
 }
 
@@ -270,23 +409,22 @@ __global__ void k_pack_weights(int n, const double* __restrict__ w, int ld, doub
 __global__ __launch_bounds__(kBlock) void k_bnd_flux(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int* __restrict__ ent, const int* __restrict__ group, const double* __restrict__ bw,
-    int ldb, const double* __restrict__ var, double* __restrict__ flux, int ld, Consts k,
-    const int* __restrict__ status)
+    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n_targets) return;
     const int n = targets[tid];
+    double U[5], acc[5];
+    load5(var, n, U);
+    load5(flux, n, acc);
     Th b;
-    thermo_exact(var[n], var[ld + n], var[2 * ld + n], var[3 * ld + n], var[4 * ld + n], b);
-    double acc[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) acc[v] = flux[v * ld + n];
+    thermo_exact(U, b);
     for (int q = ptr[tid]; q < ptr[tid + 1]; q++) {
         const int e = ent[q];
         const int g = group[e];
-        const double w0 = bw[e], w1 = bw[ldb + e], w2 = bw[2 * ldb + e];
+        const double w0 = bw[3 * (size_t)e], w1 = bw[3 * (size_t)e + 1], w2 = bw[3 * (size_t)e + 2];
         if (g <= 2) {
             // flux_boundary.elem_func:50-54
             acc[0] = acc[0] + 0.0;
@@ -304,19 +442,18 @@ __global__ __launch_bounds__(kBlock) void k_bnd_flux(
             acc[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
         }
     }
-#pragma unroll
-    for (int v = 0; v < 5; v++) flux[v * ld + n] = acc[v];
+    store5(flux, n, acc);
 }
 
 // ------------------------------------------------------------------------
-// Node-wise kernels
+// Element-wise kernels
 // ------------------------------------------------------------------------
-__global__ void k_fill5(int n, double* __restrict__ x, int ld, Consts k, int use_ff)
+__global__ void k_fill5(int n, double* __restrict__ x, Consts k, int use_ff)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
 #pragma unroll
-    for (int v = 0; v < 5; v++) x[v * ld + i] = use_ff ? k.ffv[v] : 0.0;
+    for (int v = 0; v < 5; v++) x[5 * (size_t)i + v] = use_ff ? k.ffv[v] : 0.0;
 }
 
 __global__ void k_fill(size_t n, double* __restrict__ x, double val)
@@ -331,24 +468,19 @@ __global__ void k_fill_i(size_t n, int* __restrict__ x, int val)
     if (i < n) x[i] = val;
 }
 
-__global__ void k_scale3(int n, double* __restrict__ w, int ld, double s)
+__global__ void k_scale(size_t n, double* __restrict__ w, double s)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    w[i] *= s;
-    w[ld + i] *= s;
-    w[2 * ld + i] *= s;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] *= s;
 }
 
-__global__ void k_copy5(int n, const double* __restrict__ src, double* __restrict__ dst, int ld,
-                        const int* __restrict__ status)
+__global__ void k_copy(size_t n, const double* __restrict__ src, double* __restrict__ dst,
+                       const int* __restrict__ status)
 {
     if (halted(status)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-#pragma unroll
-    for (int v = 0; v < 5; v++) dst[v * ld + i] = src[v * ld + i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // Block-wide reductions over a 256-thread workgroup (4 waves of 64).
@@ -357,7 +489,7 @@ __device__ __forceinline__ double block_min(double x)
     __shared__ double sm[kBlock / kWave];
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) {
-        double y = __shfl_down(x, o, kWave);
+        const double y = __shfl_down(x, o, kWave);
         x = (y < x) ? y : x;
     }
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
@@ -401,13 +533,16 @@ __device__ __forceinline__ int block_sum_i(int x)
     return x;
 }
 
+// Candidates for get_min_dt_kernel: like the reference's 'dt < min_dt' test
+// starting from DBL_MAX, NaN and values >= DBL_MAX never win.
+__device__ __forceinline__ double min_candidate(double d) { return (d < DBL_MAX) ? d : DBL_MAX; }
+
 // calculate_dt_kernel (time_stepping_kernels.h:13-32).  With 'old' non-null
 // it also performs copy_double_kernel (copy_double_kernel.h:6-13) and with
-// 'part' non-null it emits the per-workgroup min for get_min_dt_kernel
-// (NaN never wins, as with the reference's 'dt < min_dt' test).
+// 'part' non-null it emits the per-workgroup min for get_min_dt_kernel.
 __global__ __launch_bounds__(kBlock) void k_calc_dt(
     int n, const double* __restrict__ var, const double* __restrict__ vol,
-    double* __restrict__ dt, double* __restrict__ old, int ld, double* __restrict__ part,
+    double* __restrict__ dt, double* __restrict__ old, double* __restrict__ part,
     const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
@@ -415,18 +550,17 @@ __global__ __launch_bounds__(kBlock) void k_calc_dt(
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = DBL_MAX;
     if (i < n) {
-        const double rho = var[i], mx = var[ld + i], my = var[2 * ld + i], mz = var[3 * ld + i],
-                     E = var[4 * ld + i];
-        if (old) {
-            old[i] = rho; old[ld + i] = mx; old[2 * ld + i] = my; old[3 * ld + i] = mz;
-            old[4 * ld + i] = E;
-        }
-        const double ux = mx / rho, uy = my / rho, uz = mz / rho;
+        double U[5];
+        load5(var, i, U);
+        if (old) store5(old, i, U);
+        const double rho = U[0], E = U[4];
+        const double ux = U[1] / rho, uy = U[2] / rho, uz = U[3] / rho;
         const double ssq = ux * ux + uy * uy + uz * uz;
         const double p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
         const double c = sqrt(kGamma * p / rho);
-        d = 0.5 * (cbrt(vol[i]) / (sqrt(ssq) + c));
-        dt[i] = d;
+        const double v = 0.5 * (cbrt(vol[i]) / (sqrt(ssq) + c));
+        dt[i] = v;
+        d = min_candidate(v);
     }
     if (part) {
         const double m = block_min(d);
@@ -439,17 +573,16 @@ __global__ __launch_bounds__(kBlock) void k_min_part(int n, const double* __rest
                                                       double* __restrict__ part)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double d = (i < n) ? x[i] : DBL_MAX;
+    const double d = (i < n) ? min_candidate(x[i]) : DBL_MAX;
     const double m = block_min(d);
     if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-// Final min over the partials; slot receives min(init, partials).  With
-// 'status' set, a negative minimum raises abort code 1 (euler3d.cpp:480-484).
+// Final min over the partials; *out = min(init, partials).  With 'status'
+// set, a negative minimum raises abort code 1 (euler3d.cpp:480-484).
 __global__ __launch_bounds__(kBlock) void k_min_final(int np, const double* __restrict__ part,
                                                        double init, double* __restrict__ out,
-                                                       int* __restrict__ status,
-                                                       double* __restrict__ hist, const int* step)
+                                                       int* __restrict__ status)
 {
     if (status && *status != 0) return;
     double m = DBL_MAX;
@@ -458,7 +591,6 @@ __global__ __launch_bounds__(kBlock) void k_min_final(int np, const double* __re
     if (threadIdx.x == 0) {
         m = (m < init) ? m : init;
         *out = m;
-        if (hist && step) hist[*step] = m;
         if (status && m < 0.0) *status = 1;
     }
 }
@@ -476,7 +608,7 @@ __global__ void k_step_factor(int n, const double* __restrict__ vol, const doubl
 
 // time_step_kernel (time_stepping_kernels.h:66-86).
 __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double* __restrict__ flux,
-                            const double* __restrict__ old, double* __restrict__ var, int ld,
+                            const double* __restrict__ old, double* __restrict__ var,
                             const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
@@ -484,20 +616,22 @@ __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double factor = sf[i] / (double)(3 + 1 - rk);
+    double f[5], o[5], u[5];
+    load5(flux, i, f);
+    load5(old, i, o);
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        var[v * ld + i] = old[v * ld + i] + factor * flux[v * ld + i];
-        flux[v * ld + i] = 0.0;
-    }
+    for (int v = 0; v < 5; v++) u[v] = o[v] + factor * f[v];
+    store5(var, i, u);
+    const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    store5(flux, i, z);
 }
 
 // residual_kernel (validation.h:27-35); with 'part' set also the per-node
 // sum of squares of calc_rms_kernel (validation.h:37-44) and with 'bad' the
 // NaN/Inf count of count_bad_vals (validation.h:102-115).
 __global__ __launch_bounds__(kBlock) void k_residual(
-    int n, const double* __restrict__ old, const double* __restrict__ var,
-    double* __restrict__ res, int ld, double* __restrict__ part, int* __restrict__ bad,
-    const int* __restrict__ status)
+    int n, const double* __restrict__ old, const double* __restrict__ var, double* __restrict__ res,
+    double* __restrict__ part, int* __restrict__ bad, const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
@@ -505,14 +639,16 @@ __global__ __launch_bounds__(kBlock) void k_residual(
     double s = 0.0;
     int nb = 0;
     if (i < n) {
+        double o[5], u[5], r[5];
+        load5(old, i, o);
+        load5(var, i, u);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            const double x = var[v * ld + i];
-            const double r = x - old[v * ld + i];
-            if (res) res[v * ld + i] = r;
-            s += r * r;
-            nb += (isnan(x) || isinf(x)) ? 1 : 0;
+            r[v] = u[v] - o[v];
+            s += r[v] * r[v];
+            nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
         }
+        if (res) store5(res, i, r);
     }
     if (part) {
         const double t = block_sum(s);
@@ -525,15 +661,17 @@ __global__ __launch_bounds__(kBlock) void k_residual(
 }
 
 // calc_rms_kernel as a standalone loop (sum of squares per workgroup).
-__global__ __launch_bounds__(kBlock) void k_sumsq_part(int n, const double* __restrict__ r, int ld,
+__global__ __launch_bounds__(kBlock) void k_sumsq_part(int n, const double* __restrict__ r,
                                                         double* __restrict__ part)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (i < n) {
+        double x[5];
+        load5(r, i, x);
 #pragma unroll
-        for (int v = 0; v < 5; v++) s += r[v * ld + i] * r[v * ld + i];
+        for (int v = 0; v < 5; v++) s += x[v] * x[v];
     }
     const double t = block_sum(s);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -562,14 +700,16 @@ __global__ __launch_bounds__(kBlock) void k_sum_final(int np, const double* __re
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count_bad(int n, const double* __restrict__ x, int ld,
+__global__ __launch_bounds__(kBlock) void k_count_bad(int n, const double* __restrict__ x,
                                                        int* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 0;
     if (i < n) {
+        double u[5];
+        load5(x, i, u);
 #pragma unroll
-        for (int v = 0; v < 5; v++) c += (isnan(x[v * ld + i]) || isinf(x[v * ld + i])) ? 1 : 0;
+        for (int v = 0; v < 5; v++) c += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
     }
     c = block_sum_i(c);
     if (threadIdx.x == 0 && c) atomicAdd(out, c);
@@ -579,23 +719,26 @@ __global__ __launch_bounds__(kBlock) void k_count_bad(int n, const double* __res
 // (validation.h:91-100) when 'count' is set.
 __global__ __launch_bounds__(kBlock) void k_identify_differences(
     int n, const double* __restrict__ test, const double* __restrict__ master,
-    double* __restrict__ diff, int ld, int* __restrict__ count)
+    double* __restrict__ diff, int* __restrict__ count)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 0;
     if (i < n) {
+        double tv[5], mv[5], r[5];
+        load5(test, i, tv);
+        load5(master, i, mv);
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            double acc = master[v * ld + i] * 10.0e-8;
+            double acc = mv[v] * 10.0e-8;
             if (acc < 0.0) acc *= -1.0;
             if (acc < 3.0e-19) acc = 3.0e-19;
-            double d = test[v * ld + i] - master[v * ld + i];
+            double d = tv[v] - mv[v];
             if (d < 0.0) d *= -1.0;
-            const double r = (d > acc) ? d : 0.0;
-            if (diff) diff[v * ld + i] = r;
-            c += (r > 0.0) ? 1 : 0;
+            r[v] = (d > acc) ? d : 0.0;
+            c += (r[v] > 0.0) ? 1 : 0;
         }
+        if (diff) store5(diff, i, r);
     }
     if (count) {
         c = block_sum_i(c);
@@ -603,14 +746,16 @@ __global__ __launch_bounds__(kBlock) void k_identify_differences(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_count_pos(int n, const double* __restrict__ x, int ld,
+__global__ __launch_bounds__(kBlock) void k_count_pos(int n, const double* __restrict__ x,
                                                        int* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 0;
     if (i < n) {
+        double u[5];
+        load5(x, i, u);
 #pragma unroll
-        for (int v = 0; v < 5; v++) c += (x[v * ld + i] > 0.0) ? 1 : 0;
+        for (int v = 0; v < 5; v++) c += (u[v] > 0.0) ? 1 : 0;
     }
     c = block_sum_i(c);
     if (threadIdx.x == 0 && c) atomicAdd(out, c);
@@ -622,15 +767,15 @@ __global__ __launch_bounds__(kBlock) void k_count_pos(int n, const double* __res
 
 // up_pre_kernel: zero the coarse targets of every fine node (writes of the
 // same value from several children are benign).
-__global__ void k_up_pre(int nf, const int* __restrict__ f2c, double* __restrict__ vc, int ldc,
+__global__ void k_up_pre(int nf, const int* __restrict__ f2c, double* __restrict__ vc,
                          int* __restrict__ scratch, const int* __restrict__ status)
 {
     if (halted(status)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
     const int c = f2c[i];
-#pragma unroll
-    for (int v = 0; v < 5; v++) vc[v * ldc + c] = 0.0;
+    const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    store5(vc, c, z);
     scratch[c] = 0;
 }
 
@@ -639,9 +784,8 @@ __global__ void k_up_pre(int nf, const int* __restrict__ f2c, double* __restrict
 // folded in (one pass, same arithmetic).
 __global__ __launch_bounds__(kBlock) void k_up_gather(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
-    const int* __restrict__ child, const double* __restrict__ vf, int ldf,
-    double* __restrict__ vc, int ldc, int* __restrict__ scratch, int fused,
-    const int* __restrict__ status)
+    const int* __restrict__ child, const double* __restrict__ vf, double* __restrict__ vc,
+    int* __restrict__ scratch, int fused, const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
@@ -657,14 +801,14 @@ __global__ __launch_bounds__(kBlock) void k_up_gather(
         for (int v = 0; v < 5; v++) acc[v] = 0.0;
         cnt = 0;
     } else {
-#pragma unroll
-        for (int v = 0; v < 5; v++) acc[v] = vc[v * ldc + c];
+        load5(vc, c, acc);
         cnt = scratch[c];
     }
     for (int q = beg; q < end; q++) {
-        const int f = child[q];
+        double x[5];
+        load5(vf, child[q], x);
 #pragma unroll
-        for (int v = 0; v < 5; v++) acc[v] += vf[v * ldf + f];
+        for (int v = 0; v < 5; v++) acc[v] += x[v];
         cnt += 1;
     }
     if (fused) {
@@ -672,13 +816,12 @@ __global__ __launch_bounds__(kBlock) void k_up_gather(
 #pragma unroll
         for (int v = 0; v < 5; v++) acc[v] *= avg;
     }
-#pragma unroll
-    for (int v = 0; v < 5; v++) vc[v * ldc + c] = acc[v];
+    store5(vc, c, acc);
     scratch[c] = cnt;
 }
 
 // up_post_kernel: mean over children (count 0 keeps the value).
-__global__ void k_up_post(int nc, double* __restrict__ vc, int ldc, const int* __restrict__ scratch,
+__global__ void k_up_post(int nc, double* __restrict__ vc, const int* __restrict__ scratch,
                           const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
@@ -687,14 +830,17 @@ __global__ void k_up_post(int nc, double* __restrict__ vc, int ldc, const int* _
     if (i >= nc) return;
     const int s = scratch[i];
     const double avg = s == 0 ? 1.0 : 1.0 / (double)s;
+    double x[5];
+    load5(vc, i, x);
 #pragma unroll
-    for (int v = 0; v < 5; v++) vc[v * ldc + i] *= avg;
+    for (int v = 0; v < 5; v++) x[v] *= avg;
+    store5(vc, i, x);
 }
 
 // down_kernel (prolongation).
 __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__ vf,
-                       const double* __restrict__ rf, const double* __restrict__ xf, int ldf,
-                       const double* __restrict__ rc, const double* __restrict__ xc, int ldc,
+                       const double* __restrict__ rf, const double* __restrict__ xf,
+                       const double* __restrict__ rc, const double* __restrict__ xc,
                        const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
@@ -702,15 +848,20 @@ __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
     const int c = f2c[i];
-    const double dx = fabs(xf[i] - xc[c]);
-    const double dy = fabs(xf[ldf + i] - xc[ldc + c]);
-    const double dz = fabs(xf[2 * ldf + i] - xc[2 * ldc + c]);
+    const double dx = fabs(xf[3 * (size_t)i] - xc[3 * (size_t)c]);
+    const double dy = fabs(xf[3 * (size_t)i + 1] - xc[3 * (size_t)c + 1]);
+    const double dz = fabs(xf[3 * (size_t)i + 2] - xc[3 * (size_t)c + 2]);
     const double dm = sqrt(dx * dx + dy * dy + dz * dz);
-    vf[i] -= dm * (rc[c] - rf[i]);
-    vf[ldf + i] -= dx * (rc[ldc + c] - rf[ldf + i]);
-    vf[2 * ldf + i] -= dy * (rc[2 * ldc + c] - rf[2 * ldf + i]);
-    vf[3 * ldf + i] -= dz * (rc[3 * ldc + c] - rf[3 * ldf + i]);
-    vf[4 * ldf + i] -= dm * (rc[4 * ldc + c] - rf[4 * ldf + i]);
+    double u[5], r[5], ra[5];
+    load5(vf, i, u);
+    load5(rf, i, r);
+    load5(rc, c, ra);
+    u[0] -= dm * (ra[0] - r[0]);
+    u[1] -= dx * (ra[1] - r[1]);
+    u[2] -= dy * (ra[2] - r[2]);
+    u[3] -= dz * (ra[3] - r[3]);
+    u[4] -= dm * (ra[4] - r[4]);
+    store5(vf, i, u);
 }
 
 // ------------------------------------------------------------------------
@@ -720,7 +871,7 @@ __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__
 // calculate_cell_volumes, edge part: tetra volume per edge and the weight
 // rewrite w <- (d/|d|)|w| / |d|.
 __global__ void k_cell_volumes_edge(int ne, const int2* __restrict__ e2n, const double* __restrict__ x,
-                                    int ldn, double* __restrict__ w, int lde, double* __restrict__ tv)
+                                    double* __restrict__ w, double* __restrict__ tv)
 {
 #pragma clang fp contract(off)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -730,20 +881,21 @@ __global__ void k_cell_volumes_edge(int ne, const int2* __restrict__ e2n, const 
     double dist = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        d[i] = x[i * ldn + ab.y] - x[i * ldn + ab.x];
+        d[i] = x[3 * (size_t)ab.y + i] - x[3 * (size_t)ab.x + i];
         dist += d[i] * d[i];
     }
     dist = sqrt(dist);
+    double* we = w + 3 * (size_t)e;
     double area = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; i++) area += w[i * lde + e] * w[i * lde + e];
+    for (int i = 0; i < 3; i++) area += we[i] * we[i];
     area = sqrt(area);
     tv[e] = (1.0 / 3.0) * 0.5 * dist * area;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         double r = (d[i] / dist) * area;
         r /= dist;
-        w[i * lde + e] = r;
+        we[i] = r;
     }
 }
 

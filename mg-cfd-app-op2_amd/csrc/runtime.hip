@@ -2,9 +2,10 @@
 // MG-CFD hot path on MI355X (gfx950).  See include/mgcfd_op2.h for the ABI.
 //
 // Sets, maps and dats live on the device for their whole life.  Node dats are
-// stored SoA in an internal node order chosen for locality (reverse
-// Cuthill-McKee over the edge maps); edges are ordered by their lower
-// internal endpoint and boundary elements by their node.  Host transfers
+// stored SoA in an internal node order chosen for locality (recursive graph
+// bisection into compact parts of kPart nodes, one part per flux workgroup);
+// edges are ordered by their lower internal endpoint and boundary elements
+// by their node.  Host transfers
 // (op_fetch_data_char / op_upload_data_char) convert to and from the
 // reference's AoS layout and original element order.
 //
@@ -54,7 +55,6 @@ static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBl
 // ---------------------------------------------------------------- state --
 
 struct SetImpl {
-    int ld = 64;                 // SoA leading dimension (size rounded to 64)
     std::vector<int> perm;       // internal -> original (empty: identity)
     std::vector<int> iperm;      // original -> internal (empty: identity)
     int to_int(int orig) const { return iperm.empty() ? orig : iperm[orig]; }
@@ -72,6 +72,22 @@ struct InvPlan {
     int* d_ptr = nullptr;        // n_targets + 1
     int2* d_inc2 = nullptr;      // dim-2 maps: {other endpoint, src | col << 31}
     int* d_inc1 = nullptr;       // dim-1 maps: src
+    // blocked-gather plan (dim-2 maps onto a partitioned node set)
+    int bk_blocks = 0;           // 0 = not available
+    int bk_entries = 0;          // jagged-diagonal entries (= incidences)
+    size_t bk_lds = 0;           // dynamic LDS bytes per workgroup
+    int2* d_bk_bnode = nullptr;  // {first node, count} per block
+    int* d_bk_hptr = nullptr;    // blocks + 1
+    int* d_bk_halo = nullptr;    // halo node ids per block
+    int* d_bk_jbase = nullptr;   // first jagged entry of each block
+    int* d_bk_pbase = nullptr;   // start of each block's diagonal offsets in d_bk_jpref
+    int* d_bk_jpref = nullptr;   // per block: offset of diagonal k (k = 0..max degree)
+    unsigned* d_bk_jslot = nullptr;   // per entry: LDS slot of the other endpoint | b-role << 31
+    int* d_bk_eid = nullptr;     // per entry: internal edge id (-1 = padding)
+    double4* d_bk_w = nullptr;   // per entry: packed weights
+    const void* bk_wdat = nullptr;    // weight dat the packed weights came from
+    unsigned bk_wver = 0;
+    int bk_max_halo = 0;
 };
 
 struct MapImpl {
@@ -82,7 +98,7 @@ struct MapImpl {
 
 struct DatImpl {
     bool is_int = false;
-    void* d = nullptr;           // SoA [dim][ld]
+    void* d = nullptr;           // AoS [size][dim], internal element order
     std::vector<char> pending;   // AoS host data awaiting upload (before freeze)
     unsigned version = 1;
     double4* d_pack = nullptr;   // packed weights cache (dim-3 edge dats)
@@ -174,20 +190,24 @@ static int check_launch()
 
 // ---------------------------------------------------------- renumbering --
 
-// Reverse Cuthill-McKee ordering of a node set from all dim-2 maps into it.
-static std::vector<int> rcm_order(int n, const std::vector<const MapImpl*>& emaps,
-                                  const std::vector<int>& from_sizes)
+// Node order of a set from all dim-2 maps into it: recursive graph bisection
+// into parts of exactly kPart nodes (the last part takes the remainder).
+// Each split orders the segment by breadth-first distance from a
+// pseudo-peripheral node and cuts it at a multiple of kPart, so every part
+// is a compact, mostly connected patch of the mesh: the workgroup that owns
+// a part finds most neighbours of its nodes inside the part.
+static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>& emaps,
+                                        const std::vector<int>& from_sizes)
 {
-    std::vector<int> deg(n, 0);
+    std::vector<int> ptr(n + 1, 0);
     for (size_t m = 0; m < emaps.size(); m++) {
         const auto& h = emaps[m]->h_orig;
         for (int e = 0; e < from_sizes[m]; e++) {
-            deg[h[2 * (size_t)e]]++;
-            deg[h[2 * (size_t)e + 1]]++;
+            ptr[h[2 * (size_t)e] + 1]++;
+            ptr[h[2 * (size_t)e + 1] + 1]++;
         }
     }
-    std::vector<int> ptr(n + 1, 0);
-    for (int i = 0; i < n; i++) ptr[i + 1] = ptr[i] + deg[i];
+    for (int i = 0; i < n; i++) ptr[i + 1] += ptr[i];
     std::vector<int> adj(ptr[n]);
     std::vector<int> fill(ptr.begin(), ptr.end() - 1);
     for (size_t m = 0; m < emaps.size(); m++) {
@@ -198,48 +218,61 @@ static std::vector<int> rcm_order(int n, const std::vector<const MapImpl*>& emap
             adj[fill[b]++] = a;
         }
     }
-    std::vector<int> order;
-    order.reserve(n);
-    std::vector<char> seen(n, 0);
-    std::vector<int> level_of(n, -1);
-    auto bfs_far = [&](int s) {
-        // returns the last node of a BFS from s (a peripheral candidate)
-        std::vector<int> q{s};
-        std::vector<int> mark;
-        level_of[s] = 0;
-        mark.push_back(s);
-        int last = s;
-        for (size_t h = 0; h < q.size(); h++) {
-            int u = q[h];
-            last = u;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<int> seg(n, 0), stamp(n, 0);
+    int next_seg = 1, next_stamp = 1;
+    std::vector<int> q, comp;
+    // breadth-first order of the part of segment 'id' reachable from s,
+    // restricted to nodes not yet placed (stamp != placed)
+    auto bfs = [&](int s, int id, int mark, std::vector<int>& out) {
+        out.clear();
+        out.push_back(s);
+        stamp[s] = mark;
+        for (size_t h = 0; h < out.size(); h++) {
+            int u = out[h];
             for (int k = ptr[u]; k < ptr[u + 1]; k++) {
                 int v = adj[k];
-                if (level_of[v] < 0) { level_of[v] = level_of[u] + 1; q.push_back(v); mark.push_back(v); }
+                if (seg[v] == id && stamp[v] != mark && stamp[v] >= 0) { stamp[v] = mark; out.push_back(v); }
             }
         }
-        for (int v : mark) level_of[v] = -1;
-        return last;
     };
-    std::vector<int> byd(n);
-    std::iota(byd.begin(), byd.end(), 0);
-    std::stable_sort(byd.begin(), byd.end(), [&](int x, int y) { return deg[x] < deg[y]; });
-    std::vector<int> nb;
-    for (int s0 : byd) {
-        if (seen[s0]) continue;
-        int s = bfs_far(bfs_far(s0));
-        size_t head = order.size();
-        order.push_back(s);
-        seen[s] = 1;
-        for (size_t h = head; h < order.size(); h++) {
-            int u = order[h];
-            nb.clear();
-            for (int k = ptr[u]; k < ptr[u + 1]; k++)
-                if (!seen[adj[k]]) { seen[adj[k]] = 1; nb.push_back(adj[k]); }
-            std::stable_sort(nb.begin(), nb.end(), [&](int x, int y) { return deg[x] < deg[y]; });
-            order.insert(order.end(), nb.begin(), nb.end());
+    struct Range { int lo, hi, id; };
+    std::vector<Range> stack{{0, n, 0}};
+    std::vector<int> buf;
+    while (!stack.empty()) {
+        Range r = stack.back();
+        stack.pop_back();
+        if (r.hi - r.lo <= kPart) continue;
+        buf.clear();
+        for (int i = r.lo; i < r.hi; i++) {
+            int s0 = order[i];
+            if (stamp[s0] < 0) continue;            // already placed in this pass
+            bfs(s0, r.id, next_stamp++, comp);      // find the far end of the component
+            int far = comp.back();
+            for (int v : comp) stamp[v] = 0;
+            bfs(far, r.id, next_stamp++, comp);
+            for (int v : comp) { stamp[v] = -1; buf.push_back(v); }
         }
+        for (int v : buf) stamp[v] = 0;
+        std::copy(buf.begin(), buf.end(), order.begin() + r.lo);
+        int half = (r.hi - r.lo) / 2;
+        int mid = r.lo + ((half + kPart - 1) / kPart) * kPart;
+        if (mid >= r.hi) mid = r.hi - kPart;
+        int idl = next_seg++, idr = next_seg++;
+        for (int i = r.lo; i < mid; i++) seg[order[i]] = idl;
+        for (int i = mid; i < r.hi; i++) seg[order[i]] = idr;
+        stack.push_back({mid, r.hi, idr});
+        stack.push_back({r.lo, mid, idl});
     }
-    std::reverse(order.begin(), order.end());
+    // inside each part, nodes by decreasing degree (the blocked flux kernel
+    // walks incidences in jagged diagonals over a part's threads)
+    for (int lo = 0; lo < n; lo += kPart) {
+        const int hi = std::min(n, lo + kPart);
+        std::stable_sort(order.begin() + lo, order.begin() + hi, [&](int x, int y) {
+            return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y];
+        });
+    }
     return order;   // order[internal] = original
 }
 
@@ -274,9 +307,11 @@ static int freeze()
             if (m.to == &s && m.dim == 2 && m.from != &s) { ems.push_back(M(&m)); fsz.push_back(m.from->size); }
         if (ems.empty() || s.size == 0) continue;
         SetImpl* si = S(&s);
-        si->perm = rcm_order(s.size, ems, fsz);
-        si->iperm.assign(s.size, 0);
+        si->perm = partition_order(s.size, ems, fsz);
+        si->iperm.assign(s.size, -1);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
+        for (int i = 0; i < s.size; i++)
+            if (si->iperm[i] < 0) return fail("internal error: node order of set %s is not a permutation", s.name);
     }
     // element sets mapped into a renumbered set: order by internal targets
     for (auto& s : R.sets) {
@@ -314,28 +349,25 @@ static int freeze()
     return 0;
 }
 
-// Uploads a dat's pending AoS host data (or zeros) into its SoA buffer.
+// Uploads a dat's pending AoS host data (or zeros) into its device buffer
+// (AoS, internal element order).
 static int upload_dat(op_dat d)
 {
     DatImpl* di = D(d);
     SetImpl* si = S(d->set);
-    const int n = d->set->size, dim = d->dim, ld = si->ld;
+    const int n = d->set->size, dim = d->dim;
     const size_t esz = di->is_int ? sizeof(int) : sizeof(double);
+    const size_t bytes = esz * (size_t)dim * std::max(n, 1);
     if (!di->d) {
-        HIPCHK(hipMalloc(&di->d, esz * (size_t)dim * ld));
-        HIPCHK(hipMemset(di->d, 0, esz * (size_t)dim * ld));
+        HIPCHK(hipMalloc(&di->d, bytes));
+        HIPCHK(hipMemset(di->d, 0, bytes));
     }
     if (di->pending.empty()) return 0;
-    std::vector<char> soa(esz * (size_t)dim * ld, 0);
-    const size_t in_esz = (size_t)d->size;     // declared bytes per component
-    for (int i = 0; i < n; i++) {
-        const int o = si->to_orig(i);
-        for (int c = 0; c < dim; c++) {
-            const char* src = di->pending.data() + ((size_t)o * dim + c) * in_esz;
-            std::memcpy(soa.data() + ((size_t)c * ld + i) * esz, src, esz);
-        }
-    }
-    HIPCHK(hipMemcpy(di->d, soa.data(), soa.size(), hipMemcpyHostToDevice));
+    std::vector<char> aos(bytes, 0);
+    const size_t rec = esz * dim;
+    for (int i = 0; i < n; i++)
+        std::memcpy(aos.data() + (size_t)i * rec, di->pending.data() + (size_t)si->to_orig(i) * rec, rec);
+    HIPCHK(hipMemcpy(di->d, aos.data(), bytes, hipMemcpyHostToDevice));
     di->pending.clear();
     di->pending.shrink_to_fit();
     di->version++;
@@ -343,6 +375,106 @@ static int upload_dat(op_dat d)
 }
 
 // ---------------------------------------------------------------- plans --
+
+#define TRY(x) do { if (int rc_ = (x)) return rc_; } while (0)
+
+// Blocked-gather plan.  Blocks are the kPart-node parts of the node order
+// (halved recursively while a part's halo exceeds kHaloCap: the kernel
+// stages one halo node per thread).  Per block: the sorted halo, and the
+// incidences of its nodes in jagged-diagonal order -- diagonal k holds the
+// k-th incidence (reference edge order) of every node with degree > k;
+// nodes are sorted by decreasing degree inside a part, so diagonal k is a
+// prefix of the block's threads.
+constexpr int kHaloCap = kPart;
+
+static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
+{
+    const int nt = m->to->size;
+    std::vector<int2> blocks;
+    std::vector<int> hptr{0}, halo, jbase, pbase, jpref, eid;
+    std::vector<unsigned> jslot;
+    std::vector<int> hb;
+    int max_halo = 0;
+    std::vector<int2> work;
+    for (int lo = nt - (nt % kPart == 0 ? kPart : nt % kPart); lo >= 0; lo -= kPart)
+        work.push_back(make_int2(lo, std::min(nt, lo + kPart)));
+    while (!work.empty()) {             // LIFO over reversed parts: blocks come out in node order
+        const int lo = work.back().x, hi = work.back().y;
+        work.pop_back();
+        hb.clear();
+        for (int q = ptr[lo]; q < ptr[hi]; q++) {
+            const int o = inc2[q].x;
+            if (o < lo || o >= hi) hb.push_back(o);
+        }
+        std::sort(hb.begin(), hb.end());
+        hb.erase(std::unique(hb.begin(), hb.end()), hb.end());
+        if ((int)hb.size() > kHaloCap && hi - lo > 16) {
+            const int mid = lo + (hi - lo) / 2;
+            work.push_back(make_int2(mid, hi));
+            work.push_back(make_int2(lo, mid));
+            continue;
+        }
+        blocks.push_back(make_int2(lo, hi - lo));
+        max_halo = std::max(max_halo, (int)hb.size());
+        // degrees are non-increasing inside the block, so diagonal k covers
+        // threads [0, cnt_k)
+        const int maxdeg = ptr[lo + 1] - ptr[lo];
+        for (int t = 1; t < hi - lo; t++)
+            if (ptr[lo + t + 1] - ptr[lo + t] > ptr[lo + t] - ptr[lo + t - 1])
+                return fail("internal error: node degrees not sorted inside a part");
+        jbase.push_back((int)jslot.size());
+        pbase.push_back((int)jpref.size());
+        int off = 0;
+        for (int kd = 0; kd < maxdeg; kd++) {
+            jpref.push_back(off);
+            for (int t = 0; t < hi - lo; t++) {
+                const int nd = lo + t;
+                if (ptr[nd + 1] - ptr[nd] <= kd) break;
+                const int2 r = inc2[ptr[nd] + kd];
+                const int o = r.x;
+                const unsigned slot = (o >= lo && o < hi)
+                    ? (unsigned)(o - lo)
+                    : (unsigned)(kPart + (std::lower_bound(hb.begin(), hb.end(), o) - hb.begin()));
+                jslot.push_back(slot | ((unsigned)r.y & 0x80000000u));
+                eid.push_back(r.y & 0x7fffffff);
+                off++;
+            }
+        }
+        jpref.push_back(off);
+        halo.insert(halo.end(), hb.begin(), hb.end());
+        hptr.push_back((int)halo.size());
+    }
+    // invariants the kernel relies on
+    for (size_t bi = 0; bi < blocks.size(); bi++) {
+        const int nh = hptr[bi + 1] - hptr[bi];
+        if (blocks[bi].y > kPart || nh > kHaloCap) return fail("internal error: blocked plan block too large");
+        if (bi && blocks[bi].x != blocks[bi - 1].x + blocks[bi - 1].y) return fail("internal error: blocks not contiguous");
+    }
+    for (unsigned r : jslot) {
+        const unsigned sl = r & 0x7fffffffu;
+        if (sl >= (unsigned)(kPart + kHaloCap)) return fail("internal error: blocked plan slot out of range");
+    }
+    auto up = [](auto*& d, const auto& h) -> int {
+        using T = typename std::decay<decltype(h[0])>::type;
+        HIPCHK(hipMalloc((void**)&d, sizeof(T) * std::max<size_t>(1, h.size())));
+        if (!h.empty()) HIPCHK(hipMemcpy((void*)d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+        return 0;
+    };
+    TRY(up(P.d_bk_bnode, blocks));
+    TRY(up(P.d_bk_hptr, hptr));
+    TRY(up(P.d_bk_halo, halo));
+    TRY(up(P.d_bk_jbase, jbase));
+    TRY(up(P.d_bk_pbase, pbase));
+    TRY(up(P.d_bk_jpref, jpref));
+    TRY(up(P.d_bk_jslot, jslot));
+    TRY(up(P.d_bk_eid, eid));
+    HIPCHK(hipMalloc(&P.d_bk_w, sizeof(double4) * std::max<size_t>(1, eid.size())));
+    P.bk_blocks = (int)blocks.size();
+    P.bk_entries = (int)eid.size();
+    P.bk_max_halo = max_halo;
+    P.bk_lds = sizeof(double) * kRec * (size_t)(kPart + max_halo);
+    return 0;
+}
 
 static int build_inverse(op_map m)
 {
@@ -396,6 +528,7 @@ static int build_inverse(op_map m)
     if (dim == 2) {
         HIPCHK(hipMalloc(&P.d_inc2, sizeof(int2) * std::max<size_t>(1, inc2.size())));
         if (!inc2.empty()) HIPCHK(hipMemcpy(P.d_inc2, inc2.data(), sizeof(int2) * inc2.size(), hipMemcpyHostToDevice));
+        if (P.dense) TRY(build_blocked(P, m, ptr, inc2));
     } else {
         HIPCHK(hipMalloc(&P.d_inc1, sizeof(int) * std::max<size_t>(1, inc1.size())));
         if (!inc1.empty()) HIPCHK(hipMemcpy(P.d_inc1, inc1.data(), sizeof(int) * inc1.size(), hipMemcpyHostToDevice));
@@ -411,7 +544,7 @@ static int packed_weights(op_dat w, const double4** out)
     if (!di->d_pack) HIPCHK(hipMalloc(&di->d_pack, sizeof(double4) * std::max(1, n)));
     if (di->pack_version != di->version) {
         if (n > 0) {
-            k_pack_weights<<<nblk(n), kBlock, 0, R.stream>>>(n, (const double*)di->d, S(w->set)->ld, di->d_pack);
+            k_pack_weights<<<nblk(n), kBlock, 0, R.stream>>>(n, (const double*)di->d, di->d_pack);
             if (int rc = check_launch()) return rc;
         }
         di->pack_version = di->version;
@@ -424,7 +557,6 @@ static int packed_weights(op_dat w, const double4** out)
 
 static double* dptr(op_dat d) { return static_cast<double*>(D(d)->d); }
 static int* iptr(op_dat d) { return static_cast<int*>(D(d)->d); }
-static int ldof(op_dat d) { return S(d->set)->ld; }
 static void touch(op_dat d) { D(d)->version++; }
 
 static int need(bool ok, const char* loop, const char* what)
@@ -462,8 +594,6 @@ static int gbl(const op_arg& a, const char* type, op_access acc, const char* loo
     return need(ok, loop, what);
 }
 
-#define TRY(x) do { if (int rc_ = (x)) return rc_; } while (0)
-
 // Host readback of a device scalar, synchronising the stream.
 static int read_scalar(const double* d_src, double* out)
 {
@@ -483,27 +613,65 @@ static int read_iscalar(const int* d_src, int* out)
 
 // ------------------------------------------------------- flux launchers --
 
-static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flux, const int* status)
+// Plans and packed weights a flux launch needs; allocates and uploads, so it
+// runs outside stream capture.
+static int prepare_flux(int variant, op_map map, op_dat w)
 {
     const double4* wp = nullptr;
     TRY(packed_weights(w, &wp));
-    const int ld = ldof(var);
+    if (variant == MGCFD_FLUX_ATOMIC) return 0;
+    TRY(build_inverse(map));
+    InvPlan& P = M(map)->inv;
+    if (P.bk_blocks > 0 && (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version)) {
+        k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
+                                                                             P.d_bk_w);
+        TRY(check_launch());
+        P.bk_wdat = w;
+        P.bk_wver = D(w)->version;
+    }
+    return 0;
+}
+
+static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flux, const int* status)
+{
     if (variant == MGCFD_FLUX_ATOMIC) {
+        const double4* wp = nullptr;
+        TRY(packed_weights(w, &wp));
         const int ne = map->from->size;
         if (ne == 0) return 0;
         k_flux_atomic<<<nblk(ne), kBlock, 0, R.stream>>>(ne, (const int2*)M(map)->d_map, wp, dptr(var),
-                                                         dptr(flux), ld, R.k, status);
+                                                         dptr(flux), R.k, status);
         return check_launch();
     }
     TRY(build_inverse(map));
-    const InvPlan& P = M(map)->inv;
+    InvPlan& P = M(map)->inv;
     if (P.n_targets == 0) return 0;
-    if (variant == MGCFD_FLUX_GATHER_EXACT)
-        k_flux_gather<true><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
-            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), ld, R.k, status);
-    else
+    if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0) {
+        if (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version) {
+            k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid,
+                                                                                 dptr(w), P.d_bk_w);
+            TRY(check_launch());
+            P.bk_wdat = w;
+            P.bk_wver = D(w)->version;
+        }
+        if (variant == MGCFD_FLUX_GATHER_EXACT)
+            k_flux_blocked<true><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+                P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr,
+                P.d_bk_jslot, P.d_bk_w, dptr(var), dptr(flux), R.k, status);
+        else
+            k_flux_blocked<false><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+                P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr,
+                P.d_bk_jslot, P.d_bk_w, dptr(var), dptr(flux), R.k, status);
+        return check_launch();
+    }
+    const double4* wp = nullptr;
+    TRY(packed_weights(w, &wp));
+    if (variant == MGCFD_FLUX_GATHER_FAST)
         k_flux_gather<false><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
-            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), ld, R.k, status);
+            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status);
+    else
+        k_flux_gather<true><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
+            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status);
     return check_launch();
 }
 
@@ -521,8 +689,8 @@ static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat fl
         HIPCHK(hipMemcpy(Pm.d_targets, id.data(), sizeof(int) * id.size(), hipMemcpyHostToDevice));
     }
     k_bnd_flux<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
-                                                           iptr(group), dptr(bw), ldof(bw), dptr(var),
-                                                           dptr(flux), ldof(var), R.k, status);
+                                                           iptr(group), dptr(bw), dptr(var), dptr(flux), R.k,
+                                                           status);
     return check_launch();
 }
 
@@ -588,6 +756,10 @@ void op_exit(void)
         if (P.d_ptr) hipFree(P.d_ptr);
         if (P.d_inc2) hipFree(P.d_inc2);
         if (P.d_inc1) hipFree(P.d_inc1);
+        for (void* ptr : {(void*)P.d_bk_bnode, (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase,
+                          (void*)P.d_bk_pbase, (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid,
+                          (void*)P.d_bk_w})
+            if (ptr) hipFree(ptr);
         delete mi;
     }
     for (auto& s : R.sets) delete S(&s);
@@ -620,7 +792,6 @@ op_set op_decl_set(int size, const char* name)
     s->size = size;
     s->name = keep_name(name);
     SetImpl* si = new SetImpl();
-    si->ld = std::max(64, round_up(size, 64));
     s->impl = si;
     return s;
 }
@@ -756,16 +927,13 @@ int op_fetch_data_char(op_dat d, char* dest)
     TRY(freeze());
     DatImpl* di = D(d);
     SetImpl* si = S(d->set);
-    const int n = d->set->size, dim = d->dim, ld = si->ld;
+    const int n = d->set->size, dim = d->dim;
     const size_t esz = di->is_int ? sizeof(int) : sizeof(double);
-    std::vector<char> soa(esz * (size_t)dim * ld);
+    const size_t rec = esz * dim;
+    std::vector<char> aos(rec * (size_t)std::max(n, 1));
     HIPCHK(hipStreamSynchronize(R.stream));
-    HIPCHK(hipMemcpy(soa.data(), di->d, soa.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; i++) {
-        const int o = si->to_orig(i);
-        for (int c = 0; c < dim; c++)
-            std::memcpy(dest + ((size_t)o * dim + c) * esz, soa.data() + ((size_t)c * ld + i) * esz, esz);
-    }
+    HIPCHK(hipMemcpy(aos.data(), di->d, rec * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) std::memcpy(dest + (size_t)si->to_orig(i) * rec, aos.data() + (size_t)i * rec, rec);
     return 0;
 }
 
@@ -806,6 +974,19 @@ int op_rank(int* rank)
     return 0;
 }
 
+int mgcfd_plan_stats(op_map map, int* stats)
+{
+    if (!map || !stats) return fail("mgcfd_plan_stats: null argument");
+    TRY(build_inverse(map));
+    const InvPlan& P = M(map)->inv;
+    stats[0] = P.bk_blocks;
+    stats[1] = kPart + P.bk_max_halo;
+    stats[2] = P.bk_entries;
+    stats[3] = P.max_deg;
+    stats[4] = (int)P.bk_lds;
+    return 0;
+}
+
 int mgcfd_set_flux_variant(int v)
 {
     if (v < 0 || v >= MGCFD_FLUX_NVARIANTS) return fail("mgcfd_set_flux_variant: unknown variant %d", v);
@@ -841,7 +1022,7 @@ int op_par_loop_initialize_variables_kernel(const char* name, op_set set, op_arg
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_WRITE"));
-    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), ldof(a0.dat), R.k, 1);
+    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), R.k, 1);
     touch(a0.dat);
     return check_launch();
 }
@@ -852,7 +1033,7 @@ int op_par_loop_zero_5d_array_kernel(const char* name, op_set set, op_arg a0)
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 array[5] OP_WRITE"));
-    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), ldof(a0.dat), R.k, 0);
+    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), R.k, 0);
     touch(a0.dat);
     return check_launch();
 }
@@ -887,7 +1068,7 @@ int op_par_loop_calculate_cell_volumes(const char* name, op_set set, op_arg a0, 
     if (!wi->d_tmp) HIPCHK(hipMalloc(&wi->d_tmp, sizeof(double) * ne));
     TRY(build_inverse(a0.map));
     k_cell_volumes_edge<<<nblk(ne), kBlock, 0, R.stream>>>(ne, (const int2*)M(a0.map)->d_map, dptr(a0.dat),
-                                                           ldof(a0.dat), dptr(a2.dat), ldof(a2.dat), wi->d_tmp);
+                                                           dptr(a2.dat), wi->d_tmp);
     TRY(check_launch());
     const InvPlan& P = M(a0.map)->inv;
     k_cell_volumes_node<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr,
@@ -903,7 +1084,7 @@ int op_par_loop_dampen_ewt(const char* name, op_set set, op_arg a0)
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 3, false, L, "arg0 weights[3] OP_RW"));
-    if (set->size) k_scale3<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), ldof(a0.dat), 1e-7);
+    if (set->size) k_scale<<<nblk(3 * (size_t)set->size), kBlock, 0, R.stream>>>(3 * (size_t)set->size, dptr(a0.dat), 1e-7);
     touch(a0.dat);
     return check_launch();
 }
@@ -915,7 +1096,7 @@ int op_par_loop_copy_double_kernel(const char* name, op_set set, op_arg a0, op_a
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_READ"));
     TRY(direct(a1, set, 5, false, L, "arg1 old_variables[5] OP_WRITE"));
-    if (set->size) k_copy5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), ldof(a0.dat), nullptr);
+    if (set->size) k_copy<<<nblk(5 * (size_t)set->size), kBlock, 0, R.stream>>>(5 * (size_t)set->size, dptr(a0.dat), dptr(a1.dat), nullptr);
     touch(a1.dat);
     return check_launch();
 }
@@ -930,7 +1111,7 @@ int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_
     TRY(direct(a2, set, 1, false, L, "arg2 dt[1] OP_WRITE"));
     if (set->size)
         k_calc_dt<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
-                                                            nullptr, ldof(a0.dat), nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr);
     touch(a2.dat);
     return check_launch();
 }
@@ -949,7 +1130,7 @@ int op_par_loop_get_min_dt_kernel(const char* name, op_set set, op_arg a0, op_ar
     const int nb = (int)nblk(n);
     TRY(ensure_part(nb));
     k_min_part<<<nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), R.d_part);
-    k_min_final<<<1, kBlock, 0, R.stream>>>(nb, R.d_part, init, R.d_scal, nullptr, nullptr, nullptr);
+    k_min_final<<<1, kBlock, 0, R.stream>>>(nb, R.d_part, init, R.d_scal, nullptr);
     TRY(check_launch());
     double m;
     TRY(read_scalar(R.d_scal, &m));
@@ -1024,7 +1205,7 @@ int op_par_loop_time_step_kernel(const char* name, op_set set, op_arg a0, op_arg
     std::memcpy(&rk, a0.data, sizeof(int));
     if (set->size)
         k_time_step<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, rk, dptr(a1.dat), dptr(a2.dat), dptr(a3.dat),
-                                                              dptr(a4.dat), ldof(a2.dat), nullptr);
+                                                              dptr(a4.dat), nullptr);
     touch(a2.dat);
     touch(a4.dat);
     return check_launch();
@@ -1049,7 +1230,7 @@ int op_par_loop_unstructured_stream_kernel(const char* name, op_set set, op_arg 
     const InvPlan& P = M(a0.map)->inv;
     if (P.n_targets)
         k_stream_gather<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp,
-                                                                    dptr(a0.dat), dptr(a3.dat), ldof(a0.dat));
+                                                                    dptr(a0.dat), dptr(a3.dat));
     touch(a3.dat);
     return check_launch();
 }
@@ -1063,8 +1244,7 @@ int op_par_loop_residual_kernel(const char* name, op_set set, op_arg a0, op_arg 
     TRY(direct(a1, set, 5, false, L, "arg1 variables[5] OP_READ"));
     TRY(direct(a2, set, 5, false, L, "arg2 residuals[5] OP_WRITE"));
     if (set->size)
-        k_residual<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
-                                                             ldof(a0.dat), nullptr, nullptr, nullptr);
+        k_residual<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat), nullptr, nullptr, nullptr);
     touch(a2.dat);
     return check_launch();
 }
@@ -1082,7 +1262,7 @@ int op_par_loop_calc_rms_kernel(const char* name, op_set set, op_arg a0, op_arg 
     if (n == 0) return 0;
     const int nb = (int)nblk(n);
     TRY(ensure_part(nb));
-    k_sumsq_part<<<nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), ldof(a0.dat), R.d_part);
+    k_sumsq_part<<<nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), R.d_part);
     k_sum_final<<<1, kBlock, 0, R.stream>>>(nb, R.d_part, init, R.d_scal + 1, n, nullptr, nullptr, nullptr, nullptr);
     TRY(check_launch());
     double s;
@@ -1095,8 +1275,8 @@ static int count_loop(int n, op_dat d, int* out, int which)
 {
     k_fill_i<<<1, kBlock, 0, R.stream>>>(1, R.d_iscal, 0);
     if (n) {
-        if (which == 0) k_count_bad<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), ldof(d), R.d_iscal);
-        else k_count_pos<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), ldof(d), R.d_iscal);
+        if (which == 0) k_count_bad<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), R.d_iscal);
+        else k_count_pos<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), R.d_iscal);
         TRY(check_launch());
     }
     return read_iscalar(R.d_iscal, out);
@@ -1128,7 +1308,7 @@ int op_par_loop_up_pre_kernel(const char* name, op_set set, op_arg a0, op_arg a1
               D(a1.dat)->is_int && a1.dat->set == a0.map->to;
     TRY(need(ok, L, "arg1 up_scratch int via mg map OP_WRITE"));
     if (set->size)
-        k_up_pre<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, M(a0.map)->d_map, dptr(a0.dat), ldof(a0.dat),
+        k_up_pre<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, M(a0.map)->d_map, dptr(a0.dat),
                                                            iptr(a1.dat), nullptr);
     touch(a0.dat);
     touch(a1.dat);
@@ -1149,8 +1329,7 @@ int op_par_loop_up_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op
     const InvPlan& P = M(a1.map)->inv;
     if (P.n_targets)
         k_up_gather<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
-                                                                dptr(a0.dat), ldof(a0.dat), dptr(a1.dat),
-                                                                ldof(a1.dat), iptr(a2.dat), 0, nullptr);
+                                                                dptr(a0.dat), dptr(a1.dat), iptr(a2.dat), 0, nullptr);
     touch(a1.dat);
     touch(a2.dat);
     return check_launch();
@@ -1164,7 +1343,7 @@ int op_par_loop_up_post_kernel(const char* name, op_set set, op_arg a0, op_arg a
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_RW"));
     TRY(direct(a1, set, 1, true, L, "arg1 up_scratch int OP_READ"));
     if (set->size)
-        k_up_post<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), ldof(a0.dat), iptr(a1.dat), nullptr);
+        k_up_post<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), iptr(a1.dat), nullptr);
     touch(a0.dat);
     return check_launch();
 }
@@ -1182,8 +1361,7 @@ int op_par_loop_down_kernel(const char* name, op_set set, op_arg a0, op_arg a1, 
     TRY(need(a3.map == a4.map, L, "same mg map"));
     if (set->size)
         k_down<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, M(a3.map)->d_map, dptr(a0.dat), dptr(a1.dat),
-                                                         dptr(a2.dat), ldof(a0.dat), dptr(a3.dat), dptr(a4.dat),
-                                                         ldof(a3.dat), nullptr);
+                                                         dptr(a2.dat), dptr(a3.dat), dptr(a4.dat), nullptr);
     touch(a0.dat);
     return check_launch();
 }
@@ -1198,7 +1376,7 @@ int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op
     TRY(direct(a2, set, 5, false, L, "arg2 differences[5] OP_WRITE"));
     if (set->size)
         k_identify_differences<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat),
-                                                                         dptr(a2.dat), ldof(a0.dat), nullptr);
+                                                                         dptr(a2.dat), nullptr);
     touch(a2.dat);
     return check_launch();
 }
@@ -1217,6 +1395,53 @@ int op_par_loop_count_non_zeros(const char* name, op_set set, op_arg a0, op_arg 
     init += c;
     std::memcpy(a1.data, &init, sizeof(int));
     return 0;
+}
+
+int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a)
+{
+    if (!name) return fail("op_par_loop: null loop name");
+    struct Entry { const char* n; int k; void* f; };
+    static const Entry tab[] = {
+        {"initialize_variables_kernel", 1, (void*)op_par_loop_initialize_variables_kernel},
+        {"zero_5d_array_kernel", 1, (void*)op_par_loop_zero_5d_array_kernel},
+        {"zero_1d_array_kernel", 1, (void*)op_par_loop_zero_1d_array_kernel},
+        {"calculate_cell_volumes", 5, (void*)op_par_loop_calculate_cell_volumes},
+        {"dampen_ewt", 1, (void*)op_par_loop_dampen_ewt},
+        {"copy_double_kernel", 2, (void*)op_par_loop_copy_double_kernel},
+        {"calculate_dt_kernel", 3, (void*)op_par_loop_calculate_dt_kernel},
+        {"get_min_dt_kernel", 2, (void*)op_par_loop_get_min_dt_kernel},
+        {"compute_step_factor_kernel", 4, (void*)op_par_loop_compute_step_factor_kernel},
+        {"compute_flux_edge_kernel", 5, (void*)op_par_loop_compute_flux_edge_kernel},
+        {"compute_bnd_node_flux_kernel", 4, (void*)op_par_loop_compute_bnd_node_flux_kernel},
+        {"time_step_kernel", 5, (void*)op_par_loop_time_step_kernel},
+        {"unstructured_stream_kernel", 5, (void*)op_par_loop_unstructured_stream_kernel},
+        {"residual_kernel", 3, (void*)op_par_loop_residual_kernel},
+        {"calc_rms_kernel", 2, (void*)op_par_loop_calc_rms_kernel},
+        {"count_bad_vals", 2, (void*)op_par_loop_count_bad_vals},
+        {"up_pre_kernel", 2, (void*)op_par_loop_up_pre_kernel},
+        {"up_kernel", 3, (void*)op_par_loop_up_kernel},
+        {"up_post_kernel", 2, (void*)op_par_loop_up_post_kernel},
+        {"down_kernel", 5, (void*)op_par_loop_down_kernel},
+        {"identify_differences", 3, (void*)op_par_loop_identify_differences},
+        {"count_non_zeros", 2, (void*)op_par_loop_count_non_zeros},
+    };
+    for (const Entry& e : tab) {
+        if (std::strcmp(e.n, name) != 0) continue;
+        if (nargs != e.k) return fail("op_par_loop %s: expected %d arguments, got %d", name, e.k, nargs);
+        using F1 = int (*)(const char*, op_set, op_arg);
+        using F2 = int (*)(const char*, op_set, op_arg, op_arg);
+        using F3 = int (*)(const char*, op_set, op_arg, op_arg, op_arg);
+        using F4 = int (*)(const char*, op_set, op_arg, op_arg, op_arg, op_arg);
+        using F5 = int (*)(const char*, op_set, op_arg, op_arg, op_arg, op_arg, op_arg);
+        switch (nargs) {
+            case 1: return ((F1)e.f)(name, set, a[0]);
+            case 2: return ((F2)e.f)(name, set, a[0], a[1]);
+            case 3: return ((F3)e.f)(name, set, a[0], a[1], a[2]);
+            case 4: return ((F4)e.f)(name, set, a[0], a[1], a[2], a[3]);
+            default: return ((F5)e.f)(name, set, a[0], a[1], a[2], a[3], a[4]);
+        }
+    }
+    return fail("op_par_loop: no device implementation of loop '%s'", name);
 }
 
 }  // extern "C"

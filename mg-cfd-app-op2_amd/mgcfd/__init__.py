@@ -22,8 +22,8 @@ PKG_DIR = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgcfd.so")
 
 OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = range(6)
-FLUX_GATHER_EXACT, FLUX_GATHER_FAST, FLUX_ATOMIC = 0, 1, 2
-FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2}
+FLUX_GATHER_EXACT, FLUX_GATHER_FAST, FLUX_ATOMIC, FLUX_NODAL_EXACT = 0, 1, 2, 3
+FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2, "nodal_exact": 3}
 
 
 class OpSetCore(C.Structure):
@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
     L.mgcfd_time_flux.argtypes = [C.c_int, C.c_int, C.POINTER(OpMap), C.POINTER(OpDat), C.POINTER(OpDat),
                                   C.POINTER(OpDat), C.c_int, C.POINTER(C.c_double)]
     L.mgcfd_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.mgcfd_plan_stats.argtypes = [OpMap, C.POINTER(C.c_int)]
     L.mgcfd_mesh_box.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
     L.mgcfd_mesh_knn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
     L.mgcfd_mesh_free.argtypes = [C.POINTER(MeshC)]

@@ -83,7 +83,7 @@ def test_setup_bitwise(rt, orc, meshname, request):
 
 
 @pytest.mark.parametrize("meshname", MESHES)
-@pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic"])
+@pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic", "nodal_exact"])
 def test_flux_edge(rt, orc, meshname, variant, request):
     mesh = request.getfixturevalue(meshname)
     c = orc.freestream(0.5, 3.0)
@@ -101,7 +101,7 @@ def test_flux_edge(rt, orc, meshname, variant, request):
     g = s.state(0, "fluxes")
     o = flux0.copy()
     orc.compute_flux_edge(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o, c)
-    if variant == "gather_exact":
+    if variant in ("gather_exact", "nodal_exact"):
         assert bitwise(g, o), f"max rel {rel_err(g, o):.3e}"
     else:
         assert rel_err(g, o) <= FLUX_FAST_TOL
