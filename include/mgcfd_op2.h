@@ -201,12 +201,20 @@ int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a
  *                            (non-deterministic summation order);
  *   MGCFD_FLUX_NODAL_EXACT   node-centric gather without LDS staging, every
  *                            incidence re-evaluates the neighbour's state
- *                            (bitwise equal to the sequential backend). */
-#define MGCFD_FLUX_GATHER_EXACT 0
-#define MGCFD_FLUX_GATHER_FAST  1
-#define MGCFD_FLUX_ATOMIC       2
-#define MGCFD_FLUX_NODAL_EXACT  3
-#define MGCFD_FLUX_NVARIANTS    4
+ *                            (bitwise equal to the sequential backend);
+ *   MGCFD_FLUX_PART_COLOURED edge-parallel over mesh parts: node state and
+ *                            increments in LDS, each edge evaluated once,
+ *                            accumulation colour by colour (deterministic,
+ *                            ulp-level drift from the sequential order);
+ *   MGCFD_FLUX_PART_ATOMIC   same with LDS FP64 atomics instead of colours
+ *                            (non-deterministic summation order). */
+#define MGCFD_FLUX_GATHER_EXACT   0
+#define MGCFD_FLUX_GATHER_FAST    1
+#define MGCFD_FLUX_ATOMIC         2
+#define MGCFD_FLUX_NODAL_EXACT    3
+#define MGCFD_FLUX_PART_COLOURED  4
+#define MGCFD_FLUX_PART_ATOMIC    5
+#define MGCFD_FLUX_NVARIANTS      6
 int mgcfd_set_flux_variant(int variant);
 int mgcfd_get_flux_variant(void);
 

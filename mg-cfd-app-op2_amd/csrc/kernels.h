@@ -291,6 +291,179 @@ __global__ void k_pack_incidence_weights(int n, const int* __restrict__ eid, con
 }
 
 // ------------------------------------------------------------------------
+// Edge flux, edge-parallel over mesh parts with node state and increments
+// in LDS.  Workgroup b owns the nodes [n0, n0 + cnt) of one compact part and
+// every edge with at least one endpoint among them ("local edges": interior
+// edges once, cut edges once on each side).
+//  1. Load (all loads issued before the first barrier, flat and coalesced):
+//     owned variables, the old flux of owned nodes (into the LDS
+//     accumulators), halo variables, and this thread's local edges {slot a,
+//     slot b, owner flags, colour} with their weights.
+//  2. Each staged node's record {rho, m, E, u, p, |u| + c} is completed in
+//     LDS (one density reciprocal, fast form).
+//  3. Each thread evaluates its edges once: inc = factor (U_a - U_b) +
+//     sum_k f_k (F_a + F_b)_k with f = -w/2; node a gains +inc and node b
+//     gains -inc (flux.h:139-207 with factor_b == factor_a up to rounding).
+//  4. Accumulation into the LDS accumulators: COLOURED walks the edge
+//     colours of the part (no two edges of one colour share an owned node),
+//     one barrier per colour, so the summation order is fixed by the plan and
+//     results are deterministic; otherwise LDS FP64 atomics (ds_add_f64) in
+//     arrival order.
+//  5. The accumulators are stored as one coalesced block.
+// Not bitwise the sequential order (ulp-level drift, as the fast gather).
+// ------------------------------------------------------------------------
+constexpr int kPRec = 10;   // doubles per LDS node record of k_flux_part
+
+struct PartInfo {
+    int n0, cnt;    // owned nodes [n0, n0 + cnt)
+    int h0, nh;     // halo ids halo[h0 .. h0 + nh)
+    int e0, ne;     // local edges [e0, e0 + ne)
+    int ncol, pad;  // edge colours (COLOURED)
+};
+
+__device__ __forceinline__ void part_record(double* r)
+{
+    const double rho = r[0], mx = r[1], my = r[2], mz = r[3], E = r[4];
+    const double inv = 1.0 / rho;
+    const double ux = mx * inv, uy = my * inv, uz = mz * inv;
+    const double ssq = ux * ux + uy * uy + uz * uz;
+    const double p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
+    const double c = sqrt(kGamma * p * inv);
+    r[5] = ux; r[6] = uy; r[7] = uz; r[8] = p; r[9] = sqrt(ssq) + c;
+}
+
+// Increment of node a for edge (a, b) in the fast form; node b gains -inc.
+__device__ __forceinline__ void part_increment(const double* ra, const double* rb, double w0, double w1,
+                                               double w2, double sc, double inc[5])
+{
+    const double2* A = reinterpret_cast<const double2*>(ra);
+    const double2* B = reinterpret_cast<const double2*>(rb);
+    const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
+    const double2 b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4];
+    // a: rho a0.x, mx a0.y, my a1.x, mz a1.y, E a2.x, ux a2.y, uy a3.x, uz a3.y, p a4.x, s a4.y
+    const double ewt = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const double factor = -ewt * sc * 0.5 * (a4.y + b4.y);
+    const double fx = -0.5 * w0, fy = -0.5 * w1, fz = -0.5 * w2;
+    const double pa = a4.x, pb = b4.x;
+    const double dea = a2.x + pa, deb = b2.x + pb;
+    // momentum flux tensors F_a + F_b (symmetric)
+    const double sxx = a2.y * a0.y + pa + b2.y * b0.y + pb;
+    const double sxy = a2.y * a1.x + b2.y * b1.x;
+    const double sxz = a2.y * a1.y + b2.y * b1.y;
+    const double syy = a3.x * a1.x + pa + b3.x * b1.x + pb;
+    const double syz = a3.x * a1.y + b3.x * b1.y;
+    const double szz = a3.y * a1.y + pa + b3.y * b1.y + pb;
+    const double sdx = a2.y * dea + b2.y * deb;
+    const double sdy = a3.x * dea + b3.x * deb;
+    const double sdz = a3.y * dea + b3.y * deb;
+    inc[0] = factor * (a0.x - b0.x) + fx * (a0.y + b0.y) + fy * (a1.x + b1.x) + fz * (a1.y + b1.y);
+    inc[1] = factor * (a0.y - b0.y) + fx * sxx + fy * sxy + fz * sxz;
+    inc[2] = factor * (a1.x - b1.x) + fx * sxy + fy * syy + fz * syz;
+    inc[3] = factor * (a1.y - b1.y) + fx * sxz + fy * syz + fz * szz;
+    inc[4] = factor * (a2.x - b2.x) + fx * sdx + fy * sdy + fz * sdz;
+}
+
+__device__ __forceinline__ void part_apply(double* acc, unsigned code, const double inc[5])
+{
+    const int sa = code & 1023u, sb = (code >> 10) & 1023u;
+    if (code & (1u << 20)) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) atomicAdd(acc + sa * 5 + v, inc[v]);
+    }
+    if (code & (1u << 21)) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) atomicAdd(acc + sb * 5 + v, -inc[v]);
+    }
+}
+
+// LDS: records [(cnt + nh) * kPRec] doubles, then accumulators [cnt * 5].
+// The launch passes rec_cap = max over parts of (cnt + nh).
+template <int T, int EPT, bool COLOURED>
+__global__ __launch_bounds__(T) void k_flux_part(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const double* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
+    double* __restrict__ flux, Consts k, const int* __restrict__ status)
+{
+    extern __shared__ double sm[];
+    if (halted(status)) return;
+    const PartInfo P = parts[xcd_block()];
+    const int t = threadIdx.x;
+    double* rec = sm;
+    double* acc = sm + (size_t)rec_cap * kPRec;
+    // 1. loads
+    unsigned code[EPT];
+    double w0[EPT], w1[EPT], w2[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int j = t + i * T;
+        code[i] = 0;
+        w0[i] = w1[i] = w2[i] = 0.0;
+        if (j < P.ne) {
+            const long long e = P.e0 + j;
+            code[i] = ecode[e];
+            w0[i] = ew[e];
+            w1[i] = ew[nloc + e];
+            w2[i] = ew[2 * nloc + e];
+        }
+    }
+    const double* vo = var + (size_t)P.n0 * 5;
+    const double* fo = flux + (size_t)P.n0 * 5;
+    for (int j = t; j < P.cnt * 5; j += T) {
+        const int nd = j / 5, v = j - nd * 5;
+        rec[nd * kPRec + v] = vo[j];
+        acc[j] = fo[j];
+    }
+    for (int j = t; j < P.nh * 5; j += T) {
+        const int h = j / 5, v = j - h * 5;
+        rec[(P.cnt + h) * kPRec + v] = var[(size_t)halo[P.h0 + h] * 5 + v];
+    }
+    __syncthreads();
+    // 2. records
+    for (int s = t; s < P.cnt + P.nh; s += T) part_record(rec + s * kPRec);
+    __syncthreads();
+    // 3./4. edges
+    if constexpr (COLOURED) {
+        double inc[EPT][5];
+#pragma unroll
+        for (int i = 0; i < EPT; i++)
+            if (code[i]) part_increment(rec + (code[i] & 1023u) * kPRec, rec + ((code[i] >> 10) & 1023u) * kPRec,
+                                        w0[i], w1[i], w2[i], k.sc, inc[i]);
+        for (int c = 0; c < P.ncol; c++) {
+#pragma unroll
+            for (int i = 0; i < EPT; i++)
+                if (code[i] && (int)(code[i] >> 22) == c) part_apply(acc, code[i], inc[i]);
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            if (!code[i]) continue;
+            double inc[5];
+            part_increment(rec + (code[i] & 1023u) * kPRec, rec + ((code[i] >> 10) & 1023u) * kPRec, w0[i], w1[i],
+                           w2[i], k.sc, inc);
+            part_apply(acc, code[i], inc);
+        }
+        __syncthreads();
+    }
+    // 5. store
+    double* fw = flux + (size_t)P.n0 * 5;
+    for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
+}
+
+// Local-edge weights of k_flux_part, gathered from the weight dat into
+// three component arrays.
+__global__ void k_pack_part_weights(long long n, const int* __restrict__ eid, const double* __restrict__ w,
+                                    double* __restrict__ out)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t e = (size_t)eid[k];
+    out[k] = w[3 * e];
+    out[n + k] = w[3 * e + 1];
+    out[2 * n + k] = w[3 * e + 2];
+}
+
+// ------------------------------------------------------------------------
 // Edge flux, node-centric gather without LDS: one thread per node, every
 // incidence re-evaluates the neighbour's state from global memory.
 // ------------------------------------------------------------------------

@@ -88,6 +88,20 @@ struct InvPlan {
     const void* bk_wdat = nullptr;    // weight dat the packed weights came from
     unsigned bk_wver = 0;
     int bk_max_halo = 0;
+    // part plan (edge-parallel flux over mesh parts, k_flux_part)
+    int pt_parts = 0;            // 0 = not available
+    long long pt_nloc = 0;       // local edges over all parts
+    int pt_rec_cap = 0;          // max staged nodes (owned + halo) of one part
+    int pt_max_ne = 0;           // max local edges of one part
+    int pt_max_col = 0;
+    size_t pt_lds = 0;
+    PartInfo* d_pt_info = nullptr;
+    int* d_pt_halo = nullptr;
+    unsigned* d_pt_code = nullptr;   // slot a | slot b << 10 | own a << 20 | own b << 21 | colour << 22
+    int* d_pt_eid = nullptr;         // internal edge id per local edge
+    double* d_pt_w = nullptr;        // 3 x pt_nloc weight components
+    const void* pt_wdat = nullptr;
+    unsigned pt_wver = 0;
 };
 
 struct MapImpl {
@@ -476,6 +490,134 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
     return 0;
 }
 
+// Part plan of k_flux_part: the kPart-node parts of the node order (halved
+// while owned + halo nodes exceed kPartSlots); per part its sorted halo and
+// its local edges (edges with an owned endpoint: interior edges once, cut
+// edges on both sides), greedily coloured so that no two edges of a colour
+// share an owned node, and sorted by colour.
+constexpr int kPartSlots = 704;   // LDS node records per part (10-bit slots)
+
+static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
+{
+    const int nt = m->to->size;
+    std::vector<PartInfo> parts;
+    std::vector<int> halo, hb, eid;
+    std::vector<unsigned> code;
+    std::vector<int2> work;
+    std::vector<unsigned long long> used(kPart);   // per owned node: colours taken
+    struct LE { int a, b, e; unsigned flags; int col; };
+    std::vector<LE> le;
+    int rec_cap = 0, max_ne = 0, max_col = 0;
+    for (int lo = nt - (nt % kPart == 0 ? kPart : nt % kPart); lo >= 0; lo -= kPart)
+        work.push_back(make_int2(lo, std::min(nt, lo + kPart)));
+    while (!work.empty()) {
+        const int lo = work.back().x, hi = work.back().y;
+        work.pop_back();
+        hb.clear();
+        for (int q = ptr[lo]; q < ptr[hi]; q++) {
+            const int o = inc2[q].x;
+            if (o < lo || o >= hi) hb.push_back(o);
+        }
+        std::sort(hb.begin(), hb.end());
+        hb.erase(std::unique(hb.begin(), hb.end()), hb.end());
+        if ((hi - lo) + (int)hb.size() > kPartSlots) {
+            if (hi - lo <= 8) return fail("edge-flux part plan: node %d has more than %d neighbours", lo, kPartSlots);
+            const int mid = lo + (hi - lo) / 2;
+            work.push_back(make_int2(mid, hi));
+            work.push_back(make_int2(lo, mid));
+            continue;
+        }
+        auto slot = [&](int nd) -> int {
+            if (nd >= lo && nd < hi) return nd - lo;
+            return (hi - lo) + (int)(std::lower_bound(hb.begin(), hb.end(), nd) - hb.begin());
+        };
+        le.clear();
+        for (int nd = lo; nd < hi; nd++)
+            for (int q = ptr[nd]; q < ptr[nd + 1]; q++) {
+                const int o = inc2[q].x;
+                const bool role_b = (unsigned)inc2[q].y >> 31;
+                const int e = inc2[q].y & 0x7fffffff;
+                const bool other_owned = o >= lo && o < hi;
+                if (other_owned && role_b) continue;     // interior edge: taken from its a side
+                LE x;
+                x.e = e;
+                if (!role_b) { x.a = nd; x.b = o; x.flags = 1u | (other_owned ? 2u : 0u); }
+                else { x.a = o; x.b = nd; x.flags = 2u; }
+                le.push_back(x);
+            }
+        // greedy colouring in edge order
+        std::fill(used.begin(), used.begin() + (hi - lo), 0ull);
+        std::sort(le.begin(), le.end(), [](const LE& x, const LE& y) { return x.e < y.e; });
+        int ncol = 0;
+        for (auto& x : le) {
+            unsigned long long u = 0;
+            if (x.flags & 1u) u |= used[x.a - lo];
+            if (x.flags & 2u) u |= used[x.b - lo];
+            if (~u == 0ull) return fail("edge-flux part plan: more than 64 edge colours in a part");
+            const int c = __builtin_ctzll(~u);
+            x.col = c;
+            if (x.flags & 1u) used[x.a - lo] |= 1ull << c;
+            if (x.flags & 2u) used[x.b - lo] |= 1ull << c;
+            ncol = std::max(ncol, c + 1);
+        }
+        std::stable_sort(le.begin(), le.end(), [](const LE& x, const LE& y) { return x.col < y.col; });
+        PartInfo pi;
+        pi.n0 = lo;
+        pi.cnt = hi - lo;
+        pi.h0 = (int)halo.size();
+        pi.nh = (int)hb.size();
+        pi.e0 = (int)code.size();
+        pi.ne = (int)le.size();
+        pi.ncol = ncol;
+        pi.pad = 0;
+        for (const auto& x : le) {
+            const unsigned sa = (unsigned)slot(x.a), sb = (unsigned)slot(x.b);
+            code.push_back(sa | (sb << 10) | (x.flags << 20) | ((unsigned)x.col << 22));
+            eid.push_back(x.e);
+        }
+        halo.insert(halo.end(), hb.begin(), hb.end());
+        parts.push_back(pi);
+        rec_cap = std::max(rec_cap, pi.cnt + pi.nh);
+        max_ne = std::max(max_ne, pi.ne);
+        max_col = std::max(max_col, ncol);
+    }
+    // invariants the kernel relies on
+    long long covered = 0;
+    for (size_t i = 0; i < parts.size(); i++) {
+        const PartInfo& pi = parts[i];
+        if (pi.cnt > kPart || pi.cnt + pi.nh > kPartSlots) return fail("internal error: part plan part too large");
+        if (i && pi.n0 != parts[i - 1].n0 + parts[i - 1].cnt) return fail("internal error: parts not contiguous");
+        covered += pi.cnt;
+        for (int j = 0; j < pi.ne; j++) {
+            const unsigned c = code[pi.e0 + j];
+            if ((c & 1023u) >= (unsigned)(pi.cnt + pi.nh) || ((c >> 10) & 1023u) >= (unsigned)(pi.cnt + pi.nh))
+                return fail("internal error: part plan slot out of range");
+            if (((c & (1u << 20)) && (c & 1023u) >= (unsigned)pi.cnt) ||
+                ((c & (1u << 21)) && ((c >> 10) & 1023u) >= (unsigned)pi.cnt))
+                return fail("internal error: part plan owner flag on a halo slot");
+        }
+    }
+    if (covered != nt) return fail("internal error: part plan does not cover the node set");
+    auto up = [](auto*& d, const auto& h) -> int {
+        using T = typename std::decay<decltype(h[0])>::type;
+        HIPCHK(hipMalloc((void**)&d, sizeof(T) * std::max<size_t>(1, h.size())));
+        if (!h.empty()) HIPCHK(hipMemcpy((void*)d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+        return 0;
+    };
+    TRY(up(P.d_pt_info, parts));
+    TRY(up(P.d_pt_halo, halo));
+    TRY(up(P.d_pt_code, code));
+    TRY(up(P.d_pt_eid, eid));
+    HIPCHK(hipMalloc(&P.d_pt_w, sizeof(double) * 3 * std::max<size_t>(1, eid.size())));
+    P.pt_parts = (int)parts.size();
+    P.pt_nloc = (long long)eid.size();
+    P.pt_rec_cap = rec_cap;
+    P.pt_max_ne = max_ne;
+    P.pt_max_col = max_col;
+    P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
+    return 0;
+}
+
 static int build_inverse(op_map m)
 {
     MapImpl* mi = M(m);
@@ -528,7 +670,10 @@ static int build_inverse(op_map m)
     if (dim == 2) {
         HIPCHK(hipMalloc(&P.d_inc2, sizeof(int2) * std::max<size_t>(1, inc2.size())));
         if (!inc2.empty()) HIPCHK(hipMemcpy(P.d_inc2, inc2.data(), sizeof(int2) * inc2.size(), hipMemcpyHostToDevice));
-        if (P.dense) TRY(build_blocked(P, m, ptr, inc2));
+        if (P.dense) {
+            TRY(build_blocked(P, m, ptr, inc2));
+            TRY(build_partplan(P, m, ptr, inc2));
+        }
     } else {
         HIPCHK(hipMalloc(&P.d_inc1, sizeof(int) * std::max<size_t>(1, inc1.size())));
         if (!inc1.empty()) HIPCHK(hipMemcpy(P.d_inc1, inc1.data(), sizeof(int) * inc1.size(), hipMemcpyHostToDevice));
@@ -613,6 +758,53 @@ static int read_iscalar(const int* d_src, int* out)
 
 // ------------------------------------------------------- flux launchers --
 
+static bool part_variant(int v) { return v == MGCFD_FLUX_PART_COLOURED || v == MGCFD_FLUX_PART_ATOMIC; }
+
+// Re-gathers the plan's per-entry weights when the weight dat has changed.
+static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
+{
+    if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0 &&
+        (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version)) {
+        k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
+                                                                             P.d_bk_w);
+        TRY(check_launch());
+        P.bk_wdat = w;
+        P.bk_wver = D(w)->version;
+    }
+    if (part_variant(variant) && P.pt_parts > 0 && (P.pt_wdat != (const void*)w || P.pt_wver != D(w)->version)) {
+        k_pack_part_weights<<<nblk((size_t)P.pt_nloc), kBlock, 0, R.stream>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
+                                                                            P.d_pt_w);
+        TRY(check_launch());
+        P.pt_wdat = w;
+        P.pt_wver = D(w)->version;
+    }
+    return 0;
+}
+
+template <int T, int EPT>
+static int launch_part_t(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+{
+    if (coloured)
+        k_flux_part<T, EPT, true><<<P.pt_parts, T, P.pt_lds, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
+            status);
+    else
+        k_flux_part<T, EPT, false><<<P.pt_parts, T, P.pt_lds, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
+            status);
+    return check_launch();
+}
+
+static int launch_part(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+{
+    const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
+    if (ept <= 4) return launch_part_t<kBlock, 4>(coloured, P, var, flux, status);
+    if (ept <= 6) return launch_part_t<kBlock, 6>(coloured, P, var, flux, status);
+    if (ept <= 8) return launch_part_t<kBlock, 8>(coloured, P, var, flux, status);
+    if (ept <= 12) return launch_part_t<kBlock, 12>(coloured, P, var, flux, status);
+    return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
+}
+
 // Plans and packed weights a flux launch needs; allocates and uploads, so it
 // runs outside stream capture.
 static int prepare_flux(int variant, op_map map, op_dat w)
@@ -622,13 +814,7 @@ static int prepare_flux(int variant, op_map map, op_dat w)
     if (variant == MGCFD_FLUX_ATOMIC) return 0;
     TRY(build_inverse(map));
     InvPlan& P = M(map)->inv;
-    if (P.bk_blocks > 0 && (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version)) {
-        k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
-                                                                             P.d_bk_w);
-        TRY(check_launch());
-        P.bk_wdat = w;
-        P.bk_wver = D(w)->version;
-    }
+    TRY(refresh_plan_weights(variant, P, w));
     return 0;
 }
 
@@ -646,14 +832,12 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     TRY(build_inverse(map));
     InvPlan& P = M(map)->inv;
     if (P.n_targets == 0) return 0;
+    TRY(refresh_plan_weights(variant, P, w));
+    if (part_variant(variant)) {
+        if (P.pt_parts == 0) return fail("edge-flux variant %d needs an edge map onto every node of its set", variant);
+        return launch_part(variant == MGCFD_FLUX_PART_COLOURED, P, var, flux, status);
+    }
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0) {
-        if (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version) {
-            k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid,
-                                                                                 dptr(w), P.d_bk_w);
-            TRY(check_launch());
-            P.bk_wdat = w;
-            P.bk_wver = D(w)->version;
-        }
         if (variant == MGCFD_FLUX_GATHER_EXACT)
             k_flux_blocked<true><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
                 P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr,
@@ -758,7 +942,8 @@ void op_exit(void)
         if (P.d_inc1) hipFree(P.d_inc1);
         for (void* ptr : {(void*)P.d_bk_bnode, (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase,
                           (void*)P.d_bk_pbase, (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid,
-                          (void*)P.d_bk_w})
+                          (void*)P.d_bk_w, (void*)P.d_pt_info, (void*)P.d_pt_halo, (void*)P.d_pt_code,
+                          (void*)P.d_pt_eid, (void*)P.d_pt_w})
             if (ptr) hipFree(ptr);
         delete mi;
     }

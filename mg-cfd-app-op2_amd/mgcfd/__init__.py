@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgcfd.so")
 
 OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = range(6)
 FLUX_GATHER_EXACT, FLUX_GATHER_FAST, FLUX_ATOMIC, FLUX_NODAL_EXACT = 0, 1, 2, 3
-FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2, "nodal_exact": 3}
+FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2, "nodal_exact": 3, "part_coloured": 4,
+                 "part_atomic": 5}
 
 
 class OpSetCore(C.Structure):

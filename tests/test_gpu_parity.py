@@ -83,7 +83,8 @@ def test_setup_bitwise(rt, orc, meshname, request):
 
 
 @pytest.mark.parametrize("meshname", MESHES)
-@pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic", "nodal_exact"])
+@pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic", "nodal_exact", "part_coloured",
+                                     "part_atomic"])
 def test_flux_edge(rt, orc, meshname, variant, request):
     mesh = request.getfixturevalue(meshname)
     c = orc.freestream(0.5, 3.0)
