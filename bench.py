@@ -16,12 +16,23 @@ library's stream.
 The CPU baseline is the reference's own flux kernel (flux.h compiled in
 place by oracle/Makefile into oracle/_ref) driven in OP2 sequential order on
 one host core, timed on a bounded number of passes over the same mesh.
+
+HBM traffic per launch comes from two rocprofv3 counter runs of this script
+(FETCH_SIZE, then WRITE_SIZE), each scaled by what the same counter reads
+for the plain copy kernel (k_stream_copy) of known size that this script
+also launches:
+
+  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
+  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
+  python3 bench.py --summarize-pmc OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
 """
 import argparse
+import csv
 import ctypes as C
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -36,6 +47,16 @@ HBM_PEAK_GBS = 8000.0
 N_NODES, N_EDGES = 300_000, 930_000
 MACH, AOA = 0.5, 3.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "flux_traffic.json")
+COPY_BYTES = 512 << 20          # calibration copy: 512 MiB read + 512 MiB written per launch
+# demangled kernel name pattern of each edge-flux variant
+VARIANT_KERNELS = {
+    "gather_exact": r"k_flux_blocked<true>",
+    "gather_fast": r"k_flux_blocked<false>",
+    "atomic": r"k_flux_atomic\(",
+    "nodal_exact": r"k_flux_gather<true>",
+    "part_coloured": r"k_flux_part<\d+, \d+, true>",
+    "part_atomic": r"k_flux_part<\d+, \d+, false>",
+}
 
 
 def log(*a):
@@ -106,6 +127,44 @@ def cpu_baseline(mesh0, solver_state, budget_s):
                        f"({el:.1f} s, OP2 seq order, 1 thread, {platform.processor() or platform.machine()})")
 
 
+def pmc_means(path):
+    """Mean counter value (KiB) per kernel name of a rocprofv3 counter CSV."""
+    acc = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            k = row["Kernel_Name"]
+            v = float(row["Counter_Value"])
+            s, n = acc.get(k, (0.0, 0))
+            acc[k] = (s + v, n + 1)
+    return {k: s / n for k, (s, n) in acc.items()}
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    fetch, write = pmc_means(fetch_csv), pmc_means(write_csv)
+
+    def pick(d, pat):
+        ks = [k for k in d if re.search(pat, k)]
+        return (sum(d[k] for k in ks) / len(ks)) if ks else None
+
+    cf, cw = pick(fetch, r"k_stream_copy"), pick(write, r"k_stream_copy")
+    if not cf or not cw:
+        sys.exit("summarize-pmc: no k_stream_copy dispatch in the counter files")
+    sf, sw = COPY_BYTES / (cf * 1024), COPY_BYTES / (cw * 1024)
+    out = {"copy_kernel": {"bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
+                           "fetch_scale": round(sf, 4), "write_scale": round(sw, 4)},
+           "algorithmic_bytes": algorithmic_bytes(N_NODES, N_EDGES)}
+    for v, pat in VARIANT_KERNELS.items():
+        f, w = pick(fetch, pat), pick(write, pat)
+        if f is None or w is None:
+            continue
+        out[v] = {"fetch_kib": round(f, 1), "write_kib": round(w, 1),
+                  "traffic_bytes": round(f * 1024 * sf + w * 1024 * sw)}
+    os.makedirs(os.path.dirname(PROFILE_TRAFFIC), exist_ok=True)
+    with open(PROFILE_TRAFFIC, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,7 +173,12 @@ def main():
     ap.add_argument("--copies", type=int, default=5, help="mesh copies rotated to exceed the 256 MB cache")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -127,6 +191,10 @@ def main():
     rt = mgcfd.Runtime()
     name, ncu = rt.device_info()
     log(f"bench: device {name} ({ncu} CUs)")
+    cms = C.c_double(0)
+    mgcfd._check(rt.L.mgcfd_stream_copy(C.c_longlong(COPY_BYTES), 20, C.byref(cms)), "mgcfd_stream_copy")
+    copy_gbs = 2 * COPY_BYTES / (cms.value * 1e-3) / 1e9
+    log(f"bench: plain copy of {COPY_BYTES >> 20} MiB: {copy_gbs:.0f} GB/s (read + write)")
     solvers = []
     for k in range(args.copies):
         s = mgcfd.Solver(rt, [mesh0], MACH, AOA)
@@ -164,7 +232,8 @@ def main():
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get(best)
+            t = json.load(f).get(best)
+        traffic = t["traffic_bytes"] if t else None
     cpu = None
     if not args.no_cpu:
         var = rt.fetch(solvers[0].lv[0].variables)
@@ -188,7 +257,8 @@ def main():
         "config": {"workload": "compute_flux_edge_kernel, level 0 of the 300K-node/930K-edge kNN mesh",
                    "n_nodes": N_NODES, "n_edges": N_EDGES, "variant": best,
                    "mesh_copies_rotated": args.copies, "plan": plan,
-                   "working_set_MB": round(args.copies * nb / 1e6, 1)},
+                   "working_set_MB": round(args.copies * nb / 1e6, 1),
+                   "algorithmic_bytes_per_call": nb, "copy_kernel_GBs": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

@@ -218,9 +218,9 @@ int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a
 int mgcfd_set_flux_variant(int variant);
 int mgcfd_get_flux_variant(void);
 
-/* Edge-flux plan of an edge->node map: stats[0] workgroups, [1] staged
- * nodes per workgroup (LDS slots), [2] block-edge entries, [3] max degree,
- * [4] largest incidence count of one workgroup. */
+/* Blocked-gather plan of an edge->node map (variants GATHER_*): stats[0]
+ * workgroups, [1] staged node slots per workgroup, [2] incidences (2 per
+ * edge), [3] max node degree, [4] dynamic LDS bytes per workgroup. */
 int mgcfd_plan_stats(op_map map, int* stats);
 
 /* Last error message of this thread ("" if none). */
@@ -261,6 +261,11 @@ int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
  * cache. */
 int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
                     const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
+
+/* Times 'iters' launches of a plain device-to-device copy of 'bytes' bytes
+ * (read 'bytes', write 'bytes'); *ms_per_call receives the mean.  The
+ * calibration for achieved bandwidth and for the HBM traffic counters. */
+int mgcfd_stream_copy(long long bytes, int iters, double* ms_per_call);
 
 #ifdef __cplusplus
 }

@@ -7,7 +7,8 @@
 //   --mach M --aoa DEG     freestream (defaults: const.h:32-33, Mach 1.2, 0 deg)
 //   --device-loop          whole multigrid cycles on the device (hipGraph),
 //                          no host round trip per smoothing step
-//   --flux-variant NAME    gather_exact (default) | gather_fast | atomic | nodal_exact
+//   --flux-variant NAME    gather_exact (default) | gather_fast | atomic | nodal_exact |
+//                          part_coloured | part_atomic
 //   --validate-strict      -v with a 1e-12 relative bound per variable
 //   --print-rms            print the L0 RMS residual of every cycle
 #include <getopt.h>
@@ -258,6 +259,8 @@ int flux_variant_id(const std::string& n)
     if (n == "gather_fast") return MGCFD_FLUX_GATHER_FAST;
     if (n == "atomic") return MGCFD_FLUX_ATOMIC;
     if (n == "nodal_exact") return MGCFD_FLUX_NODAL_EXACT;
+    if (n == "part_coloured") return MGCFD_FLUX_PART_COLOURED;
+    if (n == "part_atomic") return MGCFD_FLUX_PART_ATOMIC;
     return -1;
 }
 

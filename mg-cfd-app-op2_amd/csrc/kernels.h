@@ -565,6 +565,14 @@ __global__ __launch_bounds__(kBlock) void k_stream_gather(
     store5(flux, n, acc);
 }
 
+// Plain device-to-device copy (16-byte elements, grid-stride): the
+// calibration kernel for bandwidth and for the HBM traffic counters.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(size_t n, const double2* __restrict__ a,
+                                                        double2* __restrict__ b)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) b[i] = a[i];
+}
+
 // Packed edge weights {w0, w1, w2, |w|} (flux.h:48-50).
 __global__ void k_pack_weights(int n, const double* __restrict__ w, double4* __restrict__ out)
 {

@@ -1172,6 +1172,37 @@ int mgcfd_plan_stats(op_map map, int* stats)
     return 0;
 }
 
+int mgcfd_stream_copy(long long bytes, int iters, double* ms_per_call)
+{
+    if (!R.inited) return fail("mgcfd_stream_copy: op_init has not been called");
+    if (bytes < 16 || iters < 1 || !ms_per_call) return fail("mgcfd_stream_copy: bad arguments");
+    const size_t n = (size_t)bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        hipFree(a);
+        return fail("mgcfd_stream_copy: out of device memory");
+    }
+    k_fill<<<nblk(2 * n), kBlock, 0, R.stream>>>(2 * n, reinterpret_cast<double*>(a), 1.0);
+    const unsigned grid = 256 * 32;
+    k_stream_copy<<<grid, kBlock, 0, R.stream>>>(n, a, b);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, R.stream));
+    for (int i = 0; i < iters; i++) k_stream_copy<<<grid, kBlock, 0, R.stream>>>(n, a, b);
+    HIPCHK(hipEventRecord(e1, R.stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    *ms_per_call = (double)ms / iters;
+    return check_launch();
+}
+
 int mgcfd_set_flux_variant(int v)
 {
     if (v < 0 || v >= MGCFD_FLUX_NVARIANTS) return fail("mgcfd_set_flux_variant: unknown variant %d", v);
