@@ -26,6 +26,8 @@
 
 #include <stdint.h>
 
+#include "mgcfd_mesh.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -207,14 +209,21 @@ int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a
  *                            accumulation colour by colour (deterministic,
  *                            ulp-level drift from the sequential order);
  *   MGCFD_FLUX_PART_ATOMIC   same with LDS FP64 atomics instead of colours
- *                            (non-deterministic summation order). */
+ *                            (non-deterministic summation order);
+ *   MGCFD_FLUX_MIXED_COLOURED / MGCFD_FLUX_MIXED_ATOMIC
+ *                            the part kernels with the flux arithmetic in
+ *                            FP32 (node records, weights, increments) and
+ *                            FP64 accumulation; every other loop stays FP64
+ *                            (step factor, time step, restriction sums). */
 #define MGCFD_FLUX_GATHER_EXACT   0
 #define MGCFD_FLUX_GATHER_FAST    1
 #define MGCFD_FLUX_ATOMIC         2
 #define MGCFD_FLUX_NODAL_EXACT    3
 #define MGCFD_FLUX_PART_COLOURED  4
 #define MGCFD_FLUX_PART_ATOMIC    5
-#define MGCFD_FLUX_NVARIANTS      6
+#define MGCFD_FLUX_MIXED_COLOURED 6
+#define MGCFD_FLUX_MIXED_ATOMIC   7
+#define MGCFD_FLUX_NVARIANTS      8
 int mgcfd_set_flux_variant(int variant);
 int mgcfd_get_flux_variant(void);
 
@@ -261,6 +270,64 @@ int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
  * cache. */
 int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
                     const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
+
+/*
+ * Batched solves (no counterpart in the reference, which runs one mesh per
+ * process): n independent multigrid meshes, all with the same number of
+ * levels, concatenated level by level behind offset tables and advanced
+ * together -- one launch per loop for the whole batch, one captured graph per
+ * cycle.  Each mesh has its own freestream (mach[i], aoa_deg[i]), min dt, RMS
+ * history and status; a mesh's result equals its solve alone up to the
+ * summation order of the chosen flux variant (bitwise for GATHER_EXACT).
+ * Batches run on their own HIP stream.
+ */
+typedef struct mgcfd_batch_s* mgcfd_batch;
+/* Declares the concatenated sets/maps/dats and runs the setup loops
+ * (euler3d.cpp:412-441) for every mesh. */
+int mgcfd_batch_create(int n, const mgcfd_mesh* meshes, const double* mach, const double* aoa_deg,
+                       int flux_variant, mgcfd_batch* out);
+/* Autotuner: times the level-0 flux of the batch for every candidate part
+ * size (nodes per workgroup of the part variants) and keeps the fastest for
+ * all levels; *part_nodes receives it. */
+int mgcfd_batch_autotune(mgcfd_batch b, int iters, int* part_nodes);
+/* Runs 'cycles' multigrid cycles from the setup state; rms receives
+ * [n][cycles] L0 RMS residuals and status [n] (0 ok, 1 negative min dt,
+ * 2 NaN/Inf; either may be NULL). */
+int mgcfd_batch_run(mgcfd_batch b, int cycles, int use_graph, double* rms, int* status);
+/* Copies one mesh's dat ("variables", "fluxes", "step_factors", "volumes")
+ * at one level, in the mesh's own node order. */
+int mgcfd_batch_fetch(mgcfd_batch b, int mesh, int level, const char* what, double* out);
+/* Frees the batch's device memory. */
+int mgcfd_batch_destroy(mgcfd_batch b);
+
+/*
+ * Served solves (no counterpart in the reference): a request loop with
+ * continuous batching.  Each request brings its own mesh (copied at submit),
+ * freestream, flux variant (FP64 or mixed precision), cycle cap and RMS-drop
+ * target (stop at the first cycle with rms[c] <= rms_drop * rms[0]; 0 = run
+ * to the cap).  Requests are admitted in arrival order while the estimated
+ * device memory of the admitted ones stays within hbm_budget_bytes and at
+ * most max_active run at once; each runs as its own device-resident context
+ * on its own stream, its cycles issued in chunks of cycles_per_chunk graph
+ * replays, so requests that arrive while others run overlap with them on the
+ * device.  A request's result is the same alone and in any mix.
+ * The caller drives the loop with mgcfd_server_progress (non-blocking).
+ */
+typedef struct mgcfd_server_s* mgcfd_server;
+int mgcfd_server_create(double hbm_budget_bytes, int max_active, int cycles_per_chunk, mgcfd_server* out);
+int mgcfd_server_submit(mgcfd_server s, const mgcfd_mesh* mesh, double mach, double aoa_deg, int flux_variant,
+                        int max_cycles, double rms_drop, int* request_id);
+/* Finishes, continues and admits requests; *n_active / *n_queued (nullable)
+ * receive the counts afterwards. */
+int mgcfd_server_progress(mgcfd_server s, int* n_active, int* n_queued);
+/* *state: 0 queued, 1 running, 2 done (then status -- 0 ran to the cap,
+ * 1 negative min dt, 2 NaN/Inf, 3 converged --, cycles run, latency from
+ * submit to completion in seconds, level-0 variables [N][5] and the RMS
+ * history [cycles] are filled; any output pointer may be NULL).  A request
+ * that could not be set up returns an error. */
+int mgcfd_server_result(mgcfd_server s, int request_id, int* state, int* status, int* cycles, double* latency_s,
+                        double* variables, double* rms);
+int mgcfd_server_destroy(mgcfd_server s);
 
 /* Times 'iters' launches of a plain device-to-device copy of 'bytes' bytes
  * (read 'bytes', write 'bytes'); *ms_per_call receives the mean.  The

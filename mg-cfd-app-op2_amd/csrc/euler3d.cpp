@@ -8,7 +8,7 @@
 //   --device-loop          whole multigrid cycles on the device (hipGraph),
 //                          no host round trip per smoothing step
 //   --flux-variant NAME    gather_exact (default) | gather_fast | atomic | nodal_exact |
-//                          part_coloured | part_atomic
+//                          part_coloured | part_atomic | mixed_coloured | mixed_atomic
 //   --validate-strict      -v with a 1e-12 relative bound per variable
 //   --print-rms            print the L0 RMS residual of every cycle
 #include <getopt.h>
@@ -261,6 +261,8 @@ int flux_variant_id(const std::string& n)
     if (n == "nodal_exact") return MGCFD_FLUX_NODAL_EXACT;
     if (n == "part_coloured") return MGCFD_FLUX_PART_COLOURED;
     if (n == "part_atomic") return MGCFD_FLUX_PART_ATOMIC;
+    if (n == "mixed_coloured") return MGCFD_FLUX_MIXED_COLOURED;
+    if (n == "mixed_atomic") return MGCFD_FLUX_MIXED_ATOMIC;
     return -1;
 }
 

@@ -450,6 +450,137 @@ __global__ __launch_bounds__(T) void k_flux_part(
     for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
 }
 
+// ------------------------------------------------------------------------
+// Edge flux, mixed precision: the part kernel above with the flux
+// arithmetic in FP32 (node records, weights and increments) and FP64
+// accumulation (the LDS accumulators start from the FP64 flux and receive
+// each FP32 increment converted to FP64).  Records are 12 floats:
+// {rho, mx, my, mz | E, ux, uy, uz | p, |u| + c, -, -}.
+// ------------------------------------------------------------------------
+constexpr int kPRec32 = 12;
+
+__device__ __forceinline__ void part_record32(float* r, const double* U)
+{
+    const float rho = (float)U[0], mx = (float)U[1], my = (float)U[2], mz = (float)U[3], E = (float)U[4];
+    const float inv = 1.0f / rho;
+    const float ux = mx * inv, uy = my * inv, uz = mz * inv;
+    const float ssq = ux * ux + uy * uy + uz * uz;
+    const float p = (float)(kGamma - 1.0) * (E - 0.5f * rho * ssq);
+    const float c = sqrtf((float)kGamma * p * inv);
+    float4* R4 = reinterpret_cast<float4*>(r);
+    R4[0] = make_float4(rho, mx, my, mz);
+    R4[1] = make_float4(E, ux, uy, uz);
+    R4[2] = make_float4(p, sqrtf(ssq) + c, 0.f, 0.f);
+}
+
+__device__ __forceinline__ void part_increment32(const float* ra, const float* rb, float w0, float w1, float w2,
+                                                 float sc, double inc[5])
+{
+    const float4* A = reinterpret_cast<const float4*>(ra);
+    const float4* B = reinterpret_cast<const float4*>(rb);
+    const float4 a0 = A[0], a1 = A[1], a2 = A[2], b0 = B[0], b1 = B[1], b2 = B[2];
+    // a0 {rho, mx, my, mz}, a1 {E, ux, uy, uz}, a2 {p, s}
+    const float ewt = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
+    const float factor = -ewt * sc * 0.5f * (a2.y + b2.y);
+    const float fx = -0.5f * w0, fy = -0.5f * w1, fz = -0.5f * w2;
+    const float pa = a2.x, pb = b2.x;
+    const float dea = a1.x + pa, deb = b1.x + pb;
+    const float sxx = a1.y * a0.y + pa + b1.y * b0.y + pb;
+    const float sxy = a1.y * a0.z + b1.y * b0.z;
+    const float sxz = a1.y * a0.w + b1.y * b0.w;
+    const float syy = a1.z * a0.z + pa + b1.z * b0.z + pb;
+    const float syz = a1.z * a0.w + b1.z * b0.w;
+    const float szz = a1.w * a0.w + pa + b1.w * b0.w + pb;
+    const float sdx = a1.y * dea + b1.y * deb;
+    const float sdy = a1.z * dea + b1.z * deb;
+    const float sdz = a1.w * dea + b1.w * deb;
+    inc[0] = (double)(factor * (a0.x - b0.x) + fx * (a0.y + b0.y) + fy * (a0.z + b0.z) + fz * (a0.w + b0.w));
+    inc[1] = (double)(factor * (a0.y - b0.y) + fx * sxx + fy * sxy + fz * sxz);
+    inc[2] = (double)(factor * (a0.z - b0.z) + fx * sxy + fy * syy + fz * syz);
+    inc[3] = (double)(factor * (a0.w - b0.w) + fx * sxz + fy * syz + fz * szz);
+    inc[4] = (double)(factor * (a1.x - b1.x) + fx * sdx + fy * sdy + fz * sdz);
+}
+
+// LDS: FP32 records [(cnt + nh) * kPRec32] floats (rounded up to doubles),
+// then FP64 accumulators [cnt * 5].
+template <int T, int EPT, bool COLOURED>
+__global__ __launch_bounds__(T) void k_flux_part32(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const float* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
+    double* __restrict__ flux, Consts k, const int* __restrict__ status)
+{
+    extern __shared__ double sm[];
+    if (halted(status)) return;
+    const PartInfo P = parts[xcd_block()];
+    const int t = threadIdx.x;
+    float* rec = reinterpret_cast<float*>(sm);
+    double* acc = sm + ((size_t)rec_cap * kPRec32 + 1) / 2;
+    unsigned code[EPT];
+    float w0[EPT], w1[EPT], w2[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        const int j = t + i * T;
+        code[i] = 0;
+        w0[i] = w1[i] = w2[i] = 0.f;
+        if (j < P.ne) {
+            const long long e = P.e0 + j;
+            code[i] = ecode[e];
+            w0[i] = ew[e];
+            w1[i] = ew[nloc + e];
+            w2[i] = ew[2 * nloc + e];
+        }
+    }
+    // node records straight from the FP64 state, one node per thread
+    const double* fo = flux + (size_t)P.n0 * 5;
+    for (int j = t; j < P.cnt * 5; j += T) acc[j] = fo[j];
+    for (int s2 = t; s2 < P.cnt + P.nh; s2 += T) {
+        const int nd = s2 < P.cnt ? P.n0 + s2 : halo[P.h0 + s2 - P.cnt];
+        double U[5];
+        load5(var, nd, U);
+        part_record32(rec + s2 * kPRec32, U);
+    }
+    __syncthreads();
+    const float sc = (float)k.sc;
+    if constexpr (COLOURED) {
+        double inc[EPT][5];
+#pragma unroll
+        for (int i = 0; i < EPT; i++)
+            if (code[i])
+                part_increment32(rec + (code[i] & 1023u) * kPRec32, rec + ((code[i] >> 10) & 1023u) * kPRec32,
+                                 w0[i], w1[i], w2[i], sc, inc[i]);
+        for (int c = 0; c < P.ncol; c++) {
+#pragma unroll
+            for (int i = 0; i < EPT; i++)
+                if (code[i] && (int)(code[i] >> 22) == c) part_apply(acc, code[i], inc[i]);
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < EPT; i++) {
+            if (!code[i]) continue;
+            double inc[5];
+            part_increment32(rec + (code[i] & 1023u) * kPRec32, rec + ((code[i] >> 10) & 1023u) * kPRec32, w0[i],
+                             w1[i], w2[i], sc, inc);
+            part_apply(acc, code[i], inc);
+        }
+        __syncthreads();
+    }
+    double* fw = flux + (size_t)P.n0 * 5;
+    for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
+}
+
+// FP32 local-edge weights of k_flux_part32.
+__global__ void k_pack_part_weights32(long long n, const int* __restrict__ eid, const double* __restrict__ w,
+                                      float* __restrict__ out)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t e = (size_t)eid[k];
+    out[k] = (float)w[3 * e];
+    out[n + k] = (float)w[3 * e + 1];
+    out[2 * n + k] = (float)w[3 * e + 2];
+}
+
 // Local-edge weights of k_flux_part, gathered from the weight dat into
 // three component arrays.
 __global__ void k_pack_part_weights(long long n, const int* __restrict__ eid, const double* __restrict__ w,
@@ -590,12 +721,15 @@ __global__ void k_pack_weights(int n, const double* __restrict__ w, double4* __r
 __global__ __launch_bounds__(kBlock) void k_bnd_flux(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int* __restrict__ ent, const int* __restrict__ group, const double* __restrict__ bw,
-    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
+    const double* __restrict__ var, double* __restrict__ flux, Consts k0, const int* __restrict__ status,
+    const Consts* __restrict__ kseg, const int* __restrict__ tseg)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n_targets) return;
+    // batched solves: far-field constants of the target's own mesh
+    const Consts& k = kseg ? kseg[tseg[tid]] : k0;
     const int n = targets[tid];
     double U[5], acc[5];
     load5(var, n, U);
@@ -721,6 +855,25 @@ __device__ __forceinline__ double min_candidate(double d) { return (d < DBL_MAX)
 // calculate_dt_kernel (time_stepping_kernels.h:13-32).  With 'old' non-null
 // it also performs copy_double_kernel (copy_double_kernel.h:6-13) and with
 // 'part' non-null it emits the per-workgroup min for get_min_dt_kernel.
+// dt of node i (time_stepping_kernels.h:13-32), optionally saving old = var.
+__device__ __forceinline__ double calc_dt_node(int i, const double* __restrict__ var,
+                                               const double* __restrict__ vol, double* __restrict__ dt,
+                                               double* __restrict__ old)
+{
+#pragma clang fp contract(off)
+    double U[5];
+    load5(var, i, U);
+    if (old) store5(old, i, U);
+    const double rho = U[0], E = U[4];
+    const double ux = U[1] / rho, uy = U[2] / rho, uz = U[3] / rho;
+    const double ssq = ux * ux + uy * uy + uz * uz;
+    const double p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
+    const double c = sqrt(kGamma * p / rho);
+    const double v = 0.5 * (cbrt(vol[i]) / (sqrt(ssq) + c));
+    dt[i] = v;
+    return v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_calc_dt(
     int n, const double* __restrict__ var, const double* __restrict__ vol,
     double* __restrict__ dt, double* __restrict__ old, double* __restrict__ part,
@@ -730,19 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_calc_dt(
     if (halted(status)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = DBL_MAX;
-    if (i < n) {
-        double U[5];
-        load5(var, i, U);
-        if (old) store5(old, i, U);
-        const double rho = U[0], E = U[4];
-        const double ux = U[1] / rho, uy = U[2] / rho, uz = U[3] / rho;
-        const double ssq = ux * ux + uy * uy + uz * uz;
-        const double p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
-        const double c = sqrt(kGamma * p / rho);
-        const double v = 0.5 * (cbrt(vol[i]) / (sqrt(ssq) + c));
-        dt[i] = v;
-        d = min_candidate(v);
-    }
+    if (i < n) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
     if (part) {
         const double m = block_min(d);
         if (threadIdx.x == 0) part[blockIdx.x] = m;
@@ -1095,6 +1236,122 @@ __global__ void k_cell_volumes_node(int n_targets, const int* __restrict__ targe
 }
 
 // Multigrid cycle counter, bumped once per cycle on the device.
-__global__ void k_bump(int* c) { if (threadIdx.x == 0) *c += 1; }
+// ------------------------------------------------------------------------
+// Batched solves: the per-node reductions over a segment block table
+// {segment, lo, hi} whose blocks never straddle two meshes, then one final
+// reduction block per mesh.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_calc_dt_seg(const int4* __restrict__ blk, const double* __restrict__ var,
+                                                        const double* __restrict__ vol, double* __restrict__ dt,
+                                                        double* __restrict__ old, double* __restrict__ part,
+                                                        const int* __restrict__ halt)
+{
+    if (halted(halt)) return;
+    const int4 B = blk[blockIdx.x];
+    const int i = B.y + threadIdx.x;
+    double d = DBL_MAX;
+    if (i < B.z) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
+    const double m = block_min(d);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_min_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,
+                                                          double* __restrict__ mindt, int* __restrict__ sstatus,
+                                                          const int* __restrict__ halt)
+{
+    if (halted(halt)) return;
+    const int s = blockIdx.x;
+    double m = DBL_MAX;
+    for (int i = sblk[s] + threadIdx.x; i < sblk[s + 1]; i += kBlock) m = (part[i] < m) ? part[i] : m;
+    m = block_min(m);
+    if (threadIdx.x == 0) {
+        mindt[s] = m;
+        if (m < 0.0 && sstatus[s] == 0) sstatus[s] = 1;
+    }
+}
+
+__global__ void k_step_factor_seg(const int4* __restrict__ blk, const double* __restrict__ vol,
+                                  const double* __restrict__ mindt, double* __restrict__ sf,
+                                  const int* __restrict__ halt)
+{
+    if (halted(halt)) return;
+    const int4 B = blk[blockIdx.x];
+    const int i = B.y + threadIdx.x;
+    if (i < B.z) sf[i] = mindt[B.x] / vol[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_seg(const int4* __restrict__ blk, const double* __restrict__ old,
+                                                         const double* __restrict__ var, double* __restrict__ res,
+                                                         double* __restrict__ part, int* __restrict__ badpart,
+                                                         const int* __restrict__ halt)
+{
+#pragma clang fp contract(off)
+    if (halted(halt)) return;
+    const int4 B = blk[blockIdx.x];
+    const int i = B.y + threadIdx.x;
+    double s = 0.0;
+    int nb = 0;
+    if (i < B.z) {
+        double o[5], u[5], r[5];
+        load5(old, i, o);
+        load5(var, i, u);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            r[v] = u[v] - o[v];
+            s += r[v] * r[v];
+            nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
+        }
+        store5(res, i, r);
+    }
+    const double t = block_sum(s);
+    const int tb = block_sum_i(nb);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t;
+        badpart[blockIdx.x] = tb;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,
+                                                          const int* __restrict__ badpart,
+                                                          const int* __restrict__ nnodes, double* __restrict__ hist,
+                                                          int cap, const int* __restrict__ cycle,
+                                                          int* __restrict__ sstatus, const int* __restrict__ halt)
+{
+#pragma clang fp contract(off)
+    if (halted(halt)) return;
+    const int s = blockIdx.x;
+    double x = 0.0;
+    int nb = 0;
+    for (int i = sblk[s] + threadIdx.x; i < sblk[s + 1]; i += kBlock) {
+        x += part[i];
+        nb += badpart[i];
+    }
+    x = block_sum(x);
+    nb = block_sum_i(nb);
+    if (threadIdx.x == 0) {
+        const int c = *cycle;
+        if (c < cap) hist[(size_t)s * cap + c] = sqrt(x / (double)nnodes[s]);
+        if (nb > 0 && sstatus[s] == 0) sstatus[s] = 2;
+    }
+}
+
+// Cycle counter: counts cycles that ran (a halted context stops counting).
+__global__ void k_bump(int* c, const int* __restrict__ status)
+{
+    if (threadIdx.x == 0 && !halted(status)) *c += 1;
+}
+
+// Convergence stop of served solves, run after the cycle counter moved on:
+// segment s whose L0 RMS of the cycle that just ran has dropped to 'drop'
+// times its first cycle's gets status 3, which halts every later kernel of a
+// one-segment context (the state stays that of the converged cycle).
+__global__ void k_converge(int nseg, int* __restrict__ sstatus, const double* __restrict__ rms, int cap,
+                           const int* __restrict__ cycle, double drop)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg || sstatus[s] != 0) return;
+    const int c = *cycle - 1;       // the cycle that just ran
+    if (c >= 1 && c < cap && rms[(size_t)s * cap + c] <= drop * rms[(size_t)s * cap]) sstatus[s] = 3;
+}
 
 }  // namespace mgcfd

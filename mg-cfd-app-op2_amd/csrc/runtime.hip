@@ -57,6 +57,9 @@ static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBl
 struct SetImpl {
     std::vector<int> perm;       // internal -> original (empty: identity)
     std::vector<int> iperm;      // original -> internal (empty: identity)
+    std::vector<int> seg;        // segment offsets (batched meshes), empty: one segment
+    std::vector<int> parts;      // part boundaries of the node order (size parts + 1)
+    bool frozen = false;         // internal order chosen
     int to_int(int orig) const { return iperm.empty() ? orig : iperm[orig]; }
     int to_orig(int in) const { return perm.empty() ? in : perm[in]; }
 };
@@ -102,9 +105,17 @@ struct InvPlan {
     double* d_pt_w = nullptr;        // 3 x pt_nloc weight components
     const void* pt_wdat = nullptr;
     unsigned pt_wver = 0;
+    float* d_pt_w32 = nullptr;       // same in FP32 (mixed-precision variants), allocated on first use
+    const void* pt_w32dat = nullptr;
+    unsigned pt_w32ver = 0;
+    size_t pt_lds32 = 0;
+    int pt_psize = kPart;            // max nodes per part (autotuned)
+    std::vector<int> h_ptr;          // host CSR kept for re-planning (dim-2 dense maps)
+    std::vector<int2> h_inc2;
 };
 
 struct MapImpl {
+    bool on_device = false;
     std::vector<int> h_orig;     // [from][dim], original ids, 0-based
     int* d_map = nullptr;        // [from internal][dim], internal ids
     InvPlan inv;
@@ -120,16 +131,13 @@ struct DatImpl {
     double* d_tmp = nullptr;     // per-element temporary (cell volumes)
 };
 
-struct MgLevel {
-    mgcfd_level_bind b;
-};
-
 struct Runtime {
     bool inited = false;
     int device = 0;
     hipStream_t stream = nullptr;
     int base_index = 0;
-    bool frozen = false;
+    bool frozen = false;         // at least one loop has run
+    bool dirty = false;          // objects declared since the last freeze
     std::deque<op_set_core> sets;
     std::deque<op_map_core> maps;
     std::deque<op_dat_core> dats;
@@ -143,16 +151,6 @@ struct Runtime {
     int* d_iscal = nullptr;      // [0] count
     double* h_pin = nullptr;     // pinned host landing zone
     int* h_pini = nullptr;
-    // device-resident multigrid
-    std::vector<MgLevel> mg;
-    int* d_status = nullptr;
-    int* d_cycle = nullptr;
-    int* d_bad = nullptr;
-    double* d_rms = nullptr;
-    int rms_cap = 0;
-    double* d_mindt = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
 };
 
 static Runtime R;
@@ -211,7 +209,8 @@ static int check_launch()
 // is a compact, mostly connected patch of the mesh: the workgroup that owns
 // a part finds most neighbours of its nodes inside the part.
 static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>& emaps,
-                                        const std::vector<int>& from_sizes)
+                                        const std::vector<int>& from_sizes, const std::vector<int>& segs,
+                                        std::vector<int>& parts)
 {
     std::vector<int> ptr(n + 1, 0);
     for (size_t m = 0; m < emaps.size(); m++) {
@@ -252,7 +251,13 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
         }
     };
     struct Range { int lo, hi, id; };
-    std::vector<Range> stack{{0, n, 0}};
+    // one independent bisection per segment (batched meshes never share a part)
+    std::vector<int> so = segs.empty() ? std::vector<int>{0, n} : segs;
+    std::vector<Range> stack;
+    for (size_t k = so.size() - 1; k-- > 0;) {
+        for (int i = so[k]; i < so[k + 1]; i++) seg[i] = next_seg;
+        stack.push_back({so[k], so[k + 1], next_seg++});
+    }
     std::vector<int> buf;
     while (!stack.empty()) {
         Range r = stack.back();
@@ -279,18 +284,34 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
         stack.push_back({mid, r.hi, idr});
         stack.push_back({r.lo, mid, idl});
     }
-    // inside each part, nodes by decreasing degree (the blocked flux kernel
-    // walks incidences in jagged diagonals over a part's threads)
-    for (int lo = 0; lo < n; lo += kPart) {
-        const int hi = std::min(n, lo + kPart);
-        std::stable_sort(order.begin() + lo, order.begin() + hi, [&](int x, int y) {
-            return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y];
-        });
-    }
+    // parts: kPart-node chunks of each segment; inside each part, nodes by
+    // decreasing degree (the blocked flux kernel walks incidences in jagged
+    // diagonals over a part's threads)
+    parts.assign(1, 0);
+    for (size_t k = 0; k + 1 < so.size(); k++)
+        for (int lo = so[k]; lo < so[k + 1]; lo += kPart) {
+            const int hi = std::min(so[k + 1], lo + kPart);
+            std::stable_sort(order.begin() + lo, order.begin() + hi, [&](int x, int y) {
+                return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y];
+            });
+            parts.push_back(hi);
+        }
     return order;   // order[internal] = original
 }
 
 static int upload_dat(op_dat d);
+static void mg_release_all();     // multigrid contexts and batches (mgrun.inc)
+
+// Part boundaries of a node set's internal order (kPart-node chunks when the
+// set was not partitioned).
+static std::vector<int> set_parts(op_set s)
+{
+    const SetImpl* si = static_cast<const SetImpl*>(s->impl);
+    if (!si->parts.empty()) return si->parts;
+    std::vector<int> pb{0};
+    for (int lo = 0; lo < s->size; lo += kPart) pb.push_back(std::min(s->size, lo + kPart));
+    return pb;
+}
 
 static int build_map_device(op_map m)
 {
@@ -309,19 +330,23 @@ static int build_map_device(op_map m)
     return 0;
 }
 
-// Chooses every internal order, then uploads maps and pending dats.
+// Chooses the internal order of every set declared since the last call, then
+// uploads their maps and pending dats.  Sets, maps and dats may be declared
+// at any time (batched and served solves add meshes after the first loop);
+// an order, once chosen, never changes.
 static int freeze()
 {
-    if (R.frozen) return 0;
+    if (R.frozen && !R.dirty) return 0;
     // node-like sets: targets of dim-2 maps
     for (auto& s : R.sets) {
+        if (!s.impl || S(&s)->frozen) continue;
         std::vector<const MapImpl*> ems;
         std::vector<int> fsz;
         for (auto& m : R.maps)
-            if (m.to == &s && m.dim == 2 && m.from != &s) { ems.push_back(M(&m)); fsz.push_back(m.from->size); }
+            if (m.impl && m.to == &s && m.dim == 2 && m.from != &s) { ems.push_back(M(&m)); fsz.push_back(m.from->size); }
         if (ems.empty() || s.size == 0) continue;
         SetImpl* si = S(&s);
-        si->perm = partition_order(s.size, ems, fsz);
+        si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts);
         si->iperm.assign(s.size, -1);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
         for (int i = 0; i < s.size; i++)
@@ -329,11 +354,13 @@ static int freeze()
     }
     // element sets mapped into a renumbered set: order by internal targets
     for (auto& s : R.sets) {
+        if (!s.impl) continue;
         SetImpl* si = S(&s);
-        if (!si->perm.empty() || s.size == 0) continue;
+        if (si->frozen || !si->perm.empty() || s.size == 0) continue;
         op_map key = nullptr;
         for (auto& m : R.maps)
-            if (m.from == &s && !S(m.to)->perm.empty() && (key == nullptr || m.dim > key->dim)) key = &m;
+            if (m.impl && m.from == &s && m.to->impl && !S(m.to)->perm.empty() && (key == nullptr || m.dim > key->dim))
+                key = &m;
         if (!key) continue;
         const MapImpl* km = M(key);
         const SetImpl* ts = S(key->to);
@@ -355,11 +382,18 @@ static int freeze()
         si->iperm.assign(s.size, 0);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
     }
+    for (auto& s : R.sets)
+        if (s.impl) S(&s)->frozen = true;
     R.frozen = true;
-    for (auto& m : R.maps)
+    R.dirty = false;
+    for (auto& m : R.maps) {
+        if (!m.impl || M(&m)->on_device) continue;
         if (int rc = build_map_device(&m)) return rc;
+        M(&m)->on_device = true;
+    }
     for (auto& d : R.dats)
-        if (int rc = upload_dat(&d)) return rc;
+        if (d.impl)
+            if (int rc = upload_dat(&d)) return rc;
     return 0;
 }
 
@@ -410,8 +444,8 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
     std::vector<int> hb;
     int max_halo = 0;
     std::vector<int2> work;
-    for (int lo = nt - (nt % kPart == 0 ? kPart : nt % kPart); lo >= 0; lo -= kPart)
-        work.push_back(make_int2(lo, std::min(nt, lo + kPart)));
+    const std::vector<int> pb = set_parts(m->to);
+    for (size_t k = pb.size() - 1; k-- > 0;) work.push_back(make_int2(pb[k], pb[k + 1]));
     while (!work.empty()) {             // LIFO over reversed parts: blocks come out in node order
         const int lo = work.back().x, hi = work.back().y;
         work.pop_back();
@@ -508,8 +542,8 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     struct LE { int a, b, e; unsigned flags; int col; };
     std::vector<LE> le;
     int rec_cap = 0, max_ne = 0, max_col = 0;
-    for (int lo = nt - (nt % kPart == 0 ? kPart : nt % kPart); lo >= 0; lo -= kPart)
-        work.push_back(make_int2(lo, std::min(nt, lo + kPart)));
+    const std::vector<int> pb = set_parts(m->to);
+    for (size_t k = pb.size() - 1; k-- > 0;) work.push_back(make_int2(pb[k], pb[k + 1]));
     while (!work.empty()) {
         const int lo = work.back().x, hi = work.back().y;
         work.pop_back();
@@ -520,7 +554,7 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
         }
         std::sort(hb.begin(), hb.end());
         hb.erase(std::unique(hb.begin(), hb.end()), hb.end());
-        if ((hi - lo) + (int)hb.size() > kPartSlots) {
+        if ((hi - lo) + (int)hb.size() > kPartSlots || (hi - lo) > P.pt_psize) {
             if (hi - lo <= 8) return fail("edge-flux part plan: node %d has more than %d neighbours", lo, kPartSlots);
             const int mid = lo + (hi - lo) / 2;
             work.push_back(make_int2(mid, hi));
@@ -615,6 +649,7 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     P.pt_max_ne = max_ne;
     P.pt_max_col = max_col;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
+    P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * kPRec32 + 1) / 2 + (size_t)kPart * 5);
     return 0;
 }
 
@@ -673,6 +708,8 @@ static int build_inverse(op_map m)
         if (P.dense) {
             TRY(build_blocked(P, m, ptr, inc2));
             TRY(build_partplan(P, m, ptr, inc2));
+            P.h_ptr = std::move(ptr);
+            P.h_inc2 = std::move(inc2);
         }
     } else {
         HIPCHK(hipMalloc(&P.d_inc1, sizeof(int) * std::max<size_t>(1, inc1.size())));
@@ -680,6 +717,73 @@ static int build_inverse(op_map m)
     }
     P.built = true;
     return 0;
+}
+
+static void free_partplan(InvPlan& P)
+{
+    for (void* p : {(void*)P.d_pt_info, (void*)P.d_pt_halo, (void*)P.d_pt_code, (void*)P.d_pt_eid, (void*)P.d_pt_w,
+                    (void*)P.d_pt_w32})
+        if (p) hipFree(p);
+    P.d_pt_info = nullptr; P.d_pt_halo = nullptr; P.d_pt_code = nullptr; P.d_pt_eid = nullptr;
+    P.d_pt_w = nullptr; P.d_pt_w32 = nullptr;
+    P.pt_parts = 0;
+    P.pt_wdat = P.pt_w32dat = nullptr;
+    P.pt_wver = P.pt_w32ver = 0;
+}
+
+// Re-plans the edge-parallel flux of an edge map with parts of at most
+// 'nodes' nodes (the autotuner's knob).
+static int set_part_nodes(op_map m, int nodes)
+{
+    if (nodes < 16 || nodes > kPart) return fail("set_part_nodes: %d nodes per part outside [16, %d]", nodes, kPart);
+    TRY(build_inverse(m));
+    InvPlan& P = M(m)->inv;
+    if (!P.dense || P.h_ptr.empty()) return fail("set_part_nodes: map %s has no part plan", m->name);
+    if (P.pt_psize == nodes && P.pt_parts > 0) return 0;
+    HIPCHK(hipStreamSynchronize(R.stream));
+    free_partplan(P);
+    P.pt_psize = nodes;
+    return build_partplan(P, m, P.h_ptr, P.h_inc2);
+}
+
+static void free_invplan(InvPlan& P)
+{
+    for (void* p : {(void*)P.d_targets, (void*)P.d_ptr, (void*)P.d_inc2, (void*)P.d_inc1, (void*)P.d_bk_bnode,
+                    (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase, (void*)P.d_bk_pbase,
+                    (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w})
+        if (p) hipFree(p);
+    free_partplan(P);
+    P = InvPlan();
+}
+
+// Releases the device memory and state of one set, map or dat (batched and
+// served solves that end); the handle stays valid as an empty, dead object.
+static void release_dat(op_dat d)
+{
+    if (!d || !d->impl) return;
+    DatImpl* di = D(d);
+    for (void* p : {(void*)di->d, (void*)di->d_pack, (void*)di->d_tmp})
+        if (p) hipFree(p);
+    delete di;
+    d->impl = nullptr;
+}
+
+static void release_map(op_map m)
+{
+    if (!m || !m->impl) return;
+    MapImpl* mi = M(m);
+    if (mi->d_map) hipFree(mi->d_map);
+    free_invplan(mi->inv);
+    delete mi;
+    m->impl = nullptr;
+}
+
+static void release_set(op_set s)
+{
+    if (!s || !s->impl) return;
+    delete S(s);
+    s->impl = nullptr;
+    s->size = 0;
 }
 
 static int packed_weights(op_dat w, const double4** out)
@@ -759,6 +863,7 @@ static int read_iscalar(const int* d_src, int* out)
 // ------------------------------------------------------- flux launchers --
 
 static bool part_variant(int v) { return v == MGCFD_FLUX_PART_COLOURED || v == MGCFD_FLUX_PART_ATOMIC; }
+static bool mixed_variant(int v) { return v == MGCFD_FLUX_MIXED_COLOURED || v == MGCFD_FLUX_MIXED_ATOMIC; }
 
 // Re-gathers the plan's per-entry weights when the weight dat has changed.
 static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
@@ -770,6 +875,15 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
         TRY(check_launch());
         P.bk_wdat = w;
         P.bk_wver = D(w)->version;
+    }
+    if (mixed_variant(variant) && P.pt_parts > 0 &&
+        (!P.d_pt_w32 || P.pt_w32dat != (const void*)w || P.pt_w32ver != D(w)->version)) {
+        if (!P.d_pt_w32) HIPCHK(hipMalloc(&P.d_pt_w32, sizeof(float) * 3 * std::max<size_t>(1, (size_t)P.pt_nloc)));
+        k_pack_part_weights32<<<nblk((size_t)P.pt_nloc), kBlock, 0, R.stream>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
+                                                                              P.d_pt_w32);
+        TRY(check_launch());
+        P.pt_w32dat = w;
+        P.pt_w32ver = D(w)->version;
     }
     if (part_variant(variant) && P.pt_parts > 0 && (P.pt_wdat != (const void*)w || P.pt_wver != D(w)->version)) {
         k_pack_part_weights<<<nblk((size_t)P.pt_nloc), kBlock, 0, R.stream>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
@@ -793,6 +907,30 @@ static int launch_part_t(bool coloured, const InvPlan& P, op_dat var, op_dat flu
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
             status);
     return check_launch();
+}
+
+template <int T, int EPT>
+static int launch_part32_t(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+{
+    if (coloured)
+        k_flux_part32<T, EPT, true><<<P.pt_parts, T, P.pt_lds32, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w32, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux),
+            R.k, status);
+    else
+        k_flux_part32<T, EPT, false><<<P.pt_parts, T, P.pt_lds32, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w32, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux),
+            R.k, status);
+    return check_launch();
+}
+
+static int launch_part32(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+{
+    const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
+    if (ept <= 4) return launch_part32_t<kBlock, 4>(coloured, P, var, flux, status);
+    if (ept <= 6) return launch_part32_t<kBlock, 6>(coloured, P, var, flux, status);
+    if (ept <= 8) return launch_part32_t<kBlock, 8>(coloured, P, var, flux, status);
+    if (ept <= 12) return launch_part32_t<kBlock, 12>(coloured, P, var, flux, status);
+    return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
 }
 
 static int launch_part(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
@@ -833,8 +971,9 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     InvPlan& P = M(map)->inv;
     if (P.n_targets == 0) return 0;
     TRY(refresh_plan_weights(variant, P, w));
-    if (part_variant(variant)) {
+    if (part_variant(variant) || mixed_variant(variant)) {
         if (P.pt_parts == 0) return fail("edge-flux variant %d needs an edge map onto every node of its set", variant);
+        if (mixed_variant(variant)) return launch_part32(variant == MGCFD_FLUX_MIXED_COLOURED, P, var, flux, status);
         return launch_part(variant == MGCFD_FLUX_PART_COLOURED, P, var, flux, status);
     }
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0) {
@@ -859,22 +998,31 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     return check_launch();
 }
 
-static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat flux, const int* status)
+// Identity target list of a dense boundary plan (a boundary map that
+// touches every node); the boundary kernel always walks a target list.
+static int bnd_targets(op_map b2n)
 {
     TRY(build_inverse(b2n));
+    InvPlan& P = M(b2n)->inv;
+    if (P.n_targets == 0 || P.d_targets) return 0;
+    std::vector<int> id(P.n_targets);
+    std::iota(id.begin(), id.end(), 0);
+    HIPCHK(hipMalloc(&P.d_targets, sizeof(int) * id.size()));
+    HIPCHK(hipMemcpy(P.d_targets, id.data(), sizeof(int) * id.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// kseg/tseg (batched solves): far-field constants per mesh and the mesh of
+// each boundary target; null for one mesh with the runtime's constants.
+static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat flux, const int* status,
+                      const Consts* kseg = nullptr, const int* tseg = nullptr)
+{
+    TRY(bnd_targets(b2n));
     const InvPlan& P = M(b2n)->inv;
     if (P.n_targets == 0) return 0;
-    // a dense plan has no target list; materialise identity targets lazily
-    if (P.dense && !P.d_targets) {
-        InvPlan& Pm = M(b2n)->inv;
-        std::vector<int> id(Pm.n_targets);
-        std::iota(id.begin(), id.end(), 0);
-        HIPCHK(hipMalloc(&Pm.d_targets, sizeof(int) * id.size()));
-        HIPCHK(hipMemcpy(Pm.d_targets, id.data(), sizeof(int) * id.size(), hipMemcpyHostToDevice));
-    }
     k_bnd_flux<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
                                                            iptr(group), dptr(bw), dptr(var), dptr(flux), R.k,
-                                                           status);
+                                                           status, kseg, tseg);
     return check_launch();
 }
 
@@ -907,10 +1055,6 @@ int op_init(int argc, char** argv, int diags)
     HIPCHK(hipMalloc(&R.d_iscal, 4 * sizeof(int)));
     HIPCHK(hipHostMalloc(&R.h_pin, 8 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc(&R.h_pini, 8 * sizeof(int), hipHostMallocDefault));
-    HIPCHK(hipMalloc(&R.d_status, sizeof(int)));
-    HIPCHK(hipMalloc(&R.d_cycle, sizeof(int)));
-    HIPCHK(hipMalloc(&R.d_bad, sizeof(int)));
-    HIPCHK(hipMalloc(&R.d_mindt, sizeof(double)));
     TRY(ensure_part(4096));
     default_consts();
     R.inited = true;
@@ -921,46 +1065,17 @@ void op_exit(void)
 {
     if (!R.inited) return;
     hipStreamSynchronize(R.stream);
-    if (R.graph_exec) hipGraphExecDestroy(R.graph_exec);
-    if (R.graph) hipGraphDestroy(R.graph);
-    R.graph_exec = nullptr;
-    R.graph = nullptr;
-    for (auto& d : R.dats) {
-        DatImpl* di = D(&d);
-        if (di->d) hipFree(di->d);
-        if (di->d_pack) hipFree(di->d_pack);
-        if (di->d_tmp) hipFree(di->d_tmp);
-        delete di;
-    }
-    for (auto& m : R.maps) {
-        MapImpl* mi = M(&m);
-        if (mi->d_map) hipFree(mi->d_map);
-        InvPlan& P = mi->inv;
-        if (P.d_targets) hipFree(P.d_targets);
-        if (P.d_ptr) hipFree(P.d_ptr);
-        if (P.d_inc2) hipFree(P.d_inc2);
-        if (P.d_inc1) hipFree(P.d_inc1);
-        for (void* ptr : {(void*)P.d_bk_bnode, (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase,
-                          (void*)P.d_bk_pbase, (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid,
-                          (void*)P.d_bk_w, (void*)P.d_pt_info, (void*)P.d_pt_halo, (void*)P.d_pt_code,
-                          (void*)P.d_pt_eid, (void*)P.d_pt_w})
-            if (ptr) hipFree(ptr);
-        delete mi;
-    }
+    mg_release_all();
+    for (auto& d : R.dats) release_dat(&d);
+    for (auto& m : R.maps) release_map(&m);
     for (auto& s : R.sets) delete S(&s);
     R.dats.clear();
     R.maps.clear();
     R.sets.clear();
     R.names.clear();
-    R.mg.clear();
     if (R.d_part) hipFree(R.d_part);
-    if (R.d_rms) hipFree(R.d_rms);
     hipFree(R.d_scal);
     hipFree(R.d_iscal);
-    hipFree(R.d_status);
-    hipFree(R.d_cycle);
-    hipFree(R.d_bad);
-    hipFree(R.d_mindt);
     hipHostFree(R.h_pin);
     hipHostFree(R.h_pini);
     hipStreamDestroy(R.stream);
@@ -970,7 +1085,6 @@ void op_exit(void)
 op_set op_decl_set(int size, const char* name)
 {
     if (size < 0) { fail("op_decl_set %s: negative size", name ? name : ""); return nullptr; }
-    if (R.frozen) { fail("op_decl_set %s: sets must be declared before the first loop", name ? name : ""); return nullptr; }
     R.sets.push_back(op_set_core{});
     op_set s = &R.sets.back();
     s->index = (int)R.sets.size() - 1;
@@ -978,13 +1092,13 @@ op_set op_decl_set(int size, const char* name)
     s->name = keep_name(name);
     SetImpl* si = new SetImpl();
     s->impl = si;
+    R.dirty = true;
     return s;
 }
 
 op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char* name)
 {
     if (!from || !to || dim < 1 || (!imap && from->size > 0)) { fail("op_decl_map %s: bad arguments", name ? name : ""); return nullptr; }
-    if (R.frozen) { fail("op_decl_map %s: maps must be declared before the first loop", name ? name : ""); return nullptr; }
     MapImpl* mi = new MapImpl();
     mi->h_orig.resize((size_t)from->size * dim);
     for (size_t q = 0; q < mi->h_orig.size(); q++) {
@@ -1013,6 +1127,7 @@ op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char*
     m->dim = dim;
     m->name = keep_name(name);
     m->impl = mi;
+    R.dirty = true;
     return m;
 }
 
@@ -1043,7 +1158,11 @@ static op_dat decl_dat(op_set set, int dim, const char* type, int size, const ch
     d->type = keep_name(type);
     d->name = keep_name(name);
     d->impl = di;
-    if (R.frozen && upload_dat(d)) return nullptr;
+    if (S(set)->frozen) {
+        if (upload_dat(d)) return nullptr;
+    } else {
+        R.dirty = true;
+    }
     return d;
 }
 

@@ -24,7 +24,8 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgcfd.so")
 OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = range(6)
 FLUX_GATHER_EXACT, FLUX_GATHER_FAST, FLUX_ATOMIC, FLUX_NODAL_EXACT = 0, 1, 2, 3
 FLUX_VARIANTS = {"gather_exact": 0, "gather_fast": 1, "atomic": 2, "nodal_exact": 3, "part_coloured": 4,
-                 "part_atomic": 5}
+                 "part_atomic": 5, "mixed_coloured": 6, "mixed_atomic": 7}
+FP64_FLUX_VARIANTS = {k: v for k, v in FLUX_VARIANTS.items() if not k.startswith("mixed")}
 
 
 class OpSetCore(C.Structure):
@@ -190,6 +191,33 @@ def mesh_knn(n_nodes: int, n_edges: int, levels: int = 4, ratios=None, seed: int
     return _mesh_to_numpy(p)
 
 
+def _mesh_c(mesh: list[dict], keep: list) -> MeshC:
+    """MeshC view of a list-of-levels mesh (arrays kept alive in 'keep')."""
+    m = MeshC()
+    m.levels = len(mesh)
+    for l, lv in enumerate(mesh):
+        L = m.L[l]
+        a = dict(coords=np.ascontiguousarray(lv["coords"], np.float64),
+                 e2n=np.ascontiguousarray(lv["edge_to_node"], np.int32),
+                 ew=np.ascontiguousarray(lv["edge_weights"], np.float64),
+                 b2n=np.ascontiguousarray(lv["bnd_to_node"], np.int32),
+                 bg=np.ascontiguousarray(lv["bnd_group"], np.int32),
+                 bw=np.ascontiguousarray(lv["bnd_weights"], np.float64),
+                 f2c=(np.ascontiguousarray(lv["node_to_mg_node"], np.int32)
+                      if lv.get("node_to_mg_node") is not None else None))
+        keep.append(a)
+        L.n_nodes, L.n_edges, L.n_bnd = len(a["coords"]), len(a["e2n"]), len(a["b2n"])
+        pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.coords = a["coords"].ctypes.data_as(pd)
+        L.edge_to_node = a["e2n"].ctypes.data_as(pi)
+        L.edge_weights = a["ew"].ctypes.data_as(pd)
+        L.bnd_to_node = a["b2n"].ctypes.data_as(pi)
+        L.bnd_group = a["bg"].ctypes.data_as(pi)
+        L.bnd_weights = a["bw"].ctypes.data_as(pd)
+        L.node_to_mg_node = a["f2c"].ctypes.data_as(pi) if a["f2c"] is not None else None
+    return m
+
+
 # ---------------------------------------------------------- freestream --
 
 def freestream(mach: float = 1.2, aoa_deg: float = 0.0) -> dict:
@@ -330,6 +358,92 @@ class Runtime:
 
 
 @dataclass
+class Batch:
+    """n independent multigrid solves concatenated behind offset tables and
+    advanced together (include/mgcfd_op2.h, mgcfd_batch_*)."""
+
+    def __init__(self, rt: "Runtime", meshes: list[list[dict]], mach, aoa_deg, variant: str = "gather_exact"):
+        self.rt, self.L = rt, rt.L
+        self.n = len(meshes)
+        self.sizes = [[len(lv["coords"]) for lv in m] for m in meshes]
+        keep = []
+        arr = (MeshC * self.n)(*[_mesh_c(m, keep) for m in meshes])
+        ma = (C.c_double * self.n)(*[float(x) for x in np.broadcast_to(mach, (self.n,))])
+        aa = (C.c_double * self.n)(*[float(x) for x in np.broadcast_to(aoa_deg, (self.n,))])
+        self.h = C.c_void_p()
+        _check(self.L.mgcfd_batch_create(self.n, arr, ma, aa, FLUX_VARIANTS[variant], C.byref(self.h)),
+               "mgcfd_batch_create")
+
+    def autotune(self, iters: int = 5) -> int:
+        p = C.c_int(0)
+        _check(self.L.mgcfd_batch_autotune(self.h, iters, C.byref(p)), "mgcfd_batch_autotune")
+        return p.value
+
+    def run(self, cycles: int, use_graph: bool = True):
+        rms = np.zeros((self.n, max(cycles, 1)))
+        st = np.zeros(self.n, np.int32)
+        _check(self.L.mgcfd_batch_run(self.h, cycles, int(use_graph), rms.ctypes.data_as(C.c_void_p),
+                                      st.ctypes.data_as(C.c_void_p)), "mgcfd_batch_run")
+        return st, rms[:, :cycles]
+
+    def fetch(self, mesh: int, level: int, what: str = "variables") -> np.ndarray:
+        dim = 5 if what in ("variables", "fluxes") else 1
+        out = np.zeros((self.sizes[mesh][level], dim))
+        _check(self.L.mgcfd_batch_fetch(self.h, mesh, level, what.encode(), out.ctypes.data_as(C.c_void_p)),
+               "mgcfd_batch_fetch")
+        return out
+
+    def close(self):
+        if self.h:
+            _check(self.L.mgcfd_batch_destroy(self.h), "mgcfd_batch_destroy")
+            self.h = C.c_void_p()
+
+
+class Server:
+    """Continuous-batching request loop (include/mgcfd_op2.h, mgcfd_server_*)."""
+
+    def __init__(self, rt: "Runtime", budget_bytes: float = 200e9, max_active: int = 8, chunk: int = 10):
+        self.rt, self.L = rt, rt.L
+        self.h = C.c_void_p()
+        self.n0 = {}
+        _check(self.L.mgcfd_server_create(C.c_double(budget_bytes), max_active, chunk, C.byref(self.h)),
+               "mgcfd_server_create")
+
+    def submit(self, mesh: list[dict], mach: float, aoa_deg: float, variant: str = "gather_exact",
+               max_cycles: int = 200, rms_drop: float = 0.0) -> int:
+        keep = []
+        m = _mesh_c(mesh, keep)
+        rid = C.c_int(0)
+        _check(self.L.mgcfd_server_submit(self.h, C.byref(m), C.c_double(mach), C.c_double(aoa_deg),
+                                          FLUX_VARIANTS[variant], max_cycles, C.c_double(rms_drop), C.byref(rid)),
+               "mgcfd_server_submit")
+        self.n0[rid.value] = (len(mesh[0]["coords"]), max_cycles)
+        return rid.value
+
+    def progress(self):
+        a, q = C.c_int(0), C.c_int(0)
+        _check(self.L.mgcfd_server_progress(self.h, C.byref(a), C.byref(q)), "mgcfd_server_progress")
+        return a.value, q.value
+
+    def result(self, rid: int):
+        """None while queued or running, else dict(status, cycles, latency, variables, rms)."""
+        n0, cap = self.n0[rid]
+        state, st, cyc, lat = C.c_int(0), C.c_int(0), C.c_int(0), C.c_double(0)
+        var = np.zeros((n0, 5))
+        rms = np.zeros(cap)
+        _check(self.L.mgcfd_server_result(self.h, rid, C.byref(state), C.byref(st), C.byref(cyc), C.byref(lat),
+                                          var.ctypes.data_as(C.c_void_p), rms.ctypes.data_as(C.c_void_p)),
+               "mgcfd_server_result")
+        if state.value != 2:
+            return None
+        return dict(status=st.value, cycles=cyc.value, latency=lat.value, variables=var, rms=rms[:cyc.value])
+
+    def close(self):
+        if self.h:
+            _check(self.L.mgcfd_server_destroy(self.h), "mgcfd_server_destroy")
+            self.h = C.c_void_p()
+
+
 class LevelHandles:
     """Sets, maps and dats of one multigrid level, named as euler3d.cpp names them."""
     nodes: object = None

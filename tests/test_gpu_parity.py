@@ -20,6 +20,13 @@ pytestmark = pytest.mark.gpu
 
 STATE_TOL = 1e-12        # state after N cycles, per variable
 FLUX_FAST_TOL = 1e-12    # one flux pass, fast / atomic variants
+# Mixed precision (FP32 flux arithmetic, FP64 accumulation).  One flux pass
+# is compared with the FP64 oracle relative to the largest flux of each
+# variable; the FP32 terms (~1e-7 relative) partly cancel across a node's
+# edges, so the bound is 1e-4.  The state after N cycles must stay within
+# 1e-6 relative per variable of the FP64 oracle.
+MIXED_FLUX_TOL = 1e-4
+MIXED_STATE_TOL = 1e-6
 
 
 def rel_err(g, o):
@@ -37,6 +44,7 @@ def bitwise(a, b):
 @pytest.fixture
 def rt():
     r = mgcfd.Runtime()
+    r.set_flux_variant(mgcfd.FLUX_VARIANTS["gather_exact"])   # the selection is process-wide
     yield r
     r.close()
 
@@ -84,7 +92,7 @@ def test_setup_bitwise(rt, orc, meshname, request):
 
 @pytest.mark.parametrize("meshname", MESHES)
 @pytest.mark.parametrize("variant", ["gather_exact", "gather_fast", "atomic", "nodal_exact", "part_coloured",
-                                     "part_atomic"])
+                                     "part_atomic", "mixed_coloured", "mixed_atomic"])
 def test_flux_edge(rt, orc, meshname, variant, request):
     mesh = request.getfixturevalue(meshname)
     c = orc.freestream(0.5, 3.0)
@@ -104,6 +112,10 @@ def test_flux_edge(rt, orc, meshname, variant, request):
     orc.compute_flux_edge(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o, c)
     if variant in ("gather_exact", "nodal_exact"):
         assert bitwise(g, o), f"max rel {rel_err(g, o):.3e}"
+    elif variant.startswith("mixed"):
+        err = rel_err(g - flux0, o - flux0)
+        print(f"{variant} one-pass error relative to the largest increment: {err:.3e}")
+        assert err <= MIXED_FLUX_TOL
     else:
         assert rel_err(g, o) <= FLUX_FAST_TOL
 
@@ -333,6 +345,24 @@ def test_atomic_solve_spread(rt, orc, knn_small):
     errs = [rel_err(states[l], ref["levels"][l]["variables"]) for l in range(len(knn_small))]
     print("atomic variant state spread per level:", errs)
     assert max(errs) <= STATE_TOL
+
+
+@pytest.mark.parametrize("variant", ["mixed_coloured", "mixed_atomic"])
+def test_mixed_solve_error(rt, orc, knn_small, variant):
+    """Mixed-precision solve against the FP64 oracle: per-variable state error
+    and the residual history, stated and bounded."""
+    cycles = 4
+    ref = orc.solve(knn_small, cycles, 0.5, 3.0)
+    st, rms, states = solve_gpu(rt, knn_small, cycles, 0.5, 3.0, "graph", variant=variant)
+    assert st == 0
+    for l in range(len(knn_small)):
+        o = ref["levels"][l]["variables"]
+        per_var = (np.abs(states[l] - o).max(axis=0) / np.abs(o).max(axis=0))
+        print(f"{variant} L{l} per-variable state error: {per_var}")
+        assert per_var.max() <= MIXED_STATE_TOL
+    rms_err = np.abs(rms - ref["rms"]) / np.abs(ref["rms"])
+    print(f"{variant} residual-history relative error per cycle: {rms_err}")
+    assert rms_err.max() <= 1e-3
 
 
 def test_bad_value_abort(rt, orc, box8):
