@@ -165,6 +165,105 @@ def summarize_pmc(fetch_csv, write_csv):
     print(json.dumps(out, indent=1))
 
 
+def bench_batched(rt, n_meshes, cycles, seed=40):
+    """64 independent solves (N log-uniform in [20K, 500K], E = 3.1 N, 4
+    levels, Mach U(0.3, 0.8), angle U(-2, 6) deg) batched in mixed precision
+    with the autotuned part size, against the same solves run back to back in
+    plain FP64 (GATHER_EXACT, one captured graph per solve).  Setup is
+    outside both timings."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(20_000), np.log(500_000), n_meshes)).astype(int)
+    mach = rng.uniform(0.3, 0.8, n_meshes)
+    aoa = rng.uniform(-2.0, 6.0, n_meshes)
+    t = time.perf_counter()
+    meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=1000 + i) for i, n in enumerate(sizes)]
+    log(f"bench: batched: generated {n_meshes} meshes, {int(sizes.sum())} level-0 nodes, "
+        f"{time.perf_counter() - t:.1f} s")
+    # back-to-back plain FP64 (each solve: one warm-up cycle that captures its
+    # graph, then the timed cycles)
+    t_fp64 = 0.0
+    ref0 = []
+    for i, m in enumerate(meshes):
+        b = mgcfd.Batch(rt, [m], [mach[i]], [aoa[i]], "gather_exact")
+        b.run(1)
+        t0 = time.perf_counter()
+        st, _ = b.run(cycles)
+        t_fp64 += time.perf_counter() - t0
+        ref0.append(b.fetch(0, 0))
+        b.close()
+        if (i + 1) % 16 == 0:
+            log(f"bench: batched: {i + 1}/{n_meshes} back-to-back FP64 solves timed ({t_fp64:.2f} s)")
+    # batched mixed precision, same warm-up
+    t = time.perf_counter()
+    b = mgcfd.Batch(rt, meshes, mach, aoa, "mixed_coloured")
+    part = b.autotune(5)
+    t_setup = time.perf_counter() - t
+    b.run(1)
+    t0 = time.perf_counter()
+    st, rms = b.run(cycles)
+    t_batch = time.perf_counter() - t0
+    errs = [float((np.abs(b.fetch(i, 0) - ref0[i]).max(axis=0) / np.abs(ref0[i]).max(axis=0)).max())
+            for i in range(n_meshes)]
+    b.close()
+    out = dict(n_solves=n_meshes, cycles=cycles, level0_nodes=int(sizes.sum()),
+               solves_per_s=round(n_meshes / t_batch, 3), seconds=round(t_batch, 4),
+               fp64_back_to_back_solves_per_s=round(n_meshes / t_fp64, 3),
+               fp64_back_to_back_seconds=round(t_fp64, 4), speedup=round(t_fp64 / t_batch, 3),
+               max_rel_err_vs_fp64=float(f"{max(errs):.3e}"), autotuned_part_nodes=part,
+               status_ok=bool((st == 0).all()), setup_seconds=round(t_setup, 2),
+               variant="mixed_coloured (FP32 flux, FP64 accumulation)")
+    log(f"bench: batched: {out}")
+    return out
+
+
+def bench_serving(rt, n_req, rate, seed=50):
+    """Poisson arrivals at 'rate' requests/s; N lognormal clipped to [10K, 2M]
+    nodes (4 levels); Mach/angle as batched; FP64 / mixed precision 50/50;
+    each runs to an L0 RMS of 1e-6 x its first cycle's or 200 cycles; 200 GB
+    admission budget.  Latency is submit -> result on the host."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(100_000), 1.0, n_req)), 10_000, 2_000_000).astype(int)
+    mach = rng.uniform(0.3, 0.8, n_req)
+    aoa = rng.uniform(-2.0, 6.0, n_req)
+    variant = np.where(rng.uniform(size=n_req) < 0.5, "gather_exact", "mixed_coloured")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    t = time.perf_counter()
+    meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=2000 + i) for i, n in enumerate(sizes)]
+    log(f"bench: serving: generated {n_req} meshes ({int(sizes.min())}..{int(sizes.max())} nodes), "
+        f"{time.perf_counter() - t:.1f} s")
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=10)
+    ids, done = [], {}
+    t0 = time.perf_counter()
+    k = 0
+    while len(done) < n_req:
+        now = time.perf_counter() - t0
+        while k < n_req and arrivals[k] <= now:
+            ids.append(srv.submit(meshes[k], float(mach[k]), float(aoa[k]), str(variant[k]), max_cycles=200,
+                                  rms_drop=1e-6))
+            k += 1
+        srv.progress()
+        for i in ids:
+            if i not in done:
+                r = srv.result(i)
+                if r is not None:
+                    done[i] = r
+        if now > 600:
+            srv.close()
+            raise RuntimeError(f"serving: only {len(done)} of {n_req} requests finished in 600 s")
+        time.sleep(0.0005)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([done[i]["latency"] for i in ids])
+    cyc = np.array([done[i]["cycles"] for i in ids])
+    out = dict(requests=n_req, arrival_rate_per_s=rate, wall_seconds=round(wall, 3),
+               solves_per_s=round(n_req / wall, 3), p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+               p99_latency_s=round(float(np.percentile(lat, 99)), 4), mean_cycles=float(cyc.mean()),
+               nodes_min=int(sizes.min()), nodes_max=int(sizes.max()),
+               status_ok=all(done[i]["status"] in (0, 3) for i in ids))
+    log(f"bench: serving: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +272,12 @@ def main():
     ap.add_argument("--copies", type=int, default=5, help="mesh copies rotated to exceed the 256 MB cache")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-solves measurement")
+    ap.add_argument("--batch-meshes", type=int, default=64)
+    ap.add_argument("--batch-cycles", type=int, default=30)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
     args = ap.parse_args()
@@ -239,6 +344,8 @@ def main():
         var = rt.fetch(solvers[0].lv[0].variables)
         w = rt.fetch(solvers[0].lv[0].edge_weights)
         cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds)
+    batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles)
+    serving = None if args.no_serve else bench_serving(rt, args.serve_requests, args.serve_rate)
     rt.close()
 
     out = {
@@ -262,6 +369,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
         "variants": {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in r2.items()}
                      for k, r2 in results.items()},
     }
