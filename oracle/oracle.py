@@ -144,6 +144,20 @@ class Oracle:
     def count_non_zeros(self, diff) -> int:
         return self.fn("count_non_zeros")(len(diff), _p(diff))
 
+    # --- multi-threaded CPU baseline (reference kernel, coloured edges; 'ref' only)
+    def colour_edges(self, e2n, n_nodes, max_colours=64):
+        order = np.zeros(len(e2n), np.int32)
+        cptr = np.zeros(max_colours + 1, np.int32)
+        nc = self.fn("colour_edges")(len(e2n), n_nodes, _p(e2n), max_colours, _p(order), _p(cptr))
+        if nc < 0:
+            raise RuntimeError("edge colouring needs more than %d colours" % max_colours)
+        return order, cptr[:nc + 1].copy()
+
+    def compute_flux_edge_omp(self, colouring, e2n, var, ew, flux, c, threads):
+        order, cptr = colouring
+        self.fn("compute_flux_edge_omp")(len(cptr) - 1, _p(cptr), _p(order), _p(e2n), _p(var), _p(ew), _p(flux),
+                                         C.byref(c), threads)
+
     # --- whole solve
     def solve(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, consts: Consts | None = None,
               after_setup=None):

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <sstream>
+#include <vector>
 
 #include "const.h"
 #include "structures.h"
@@ -139,6 +140,53 @@ void ref_compute_flux_edge(int n_edges, const int* e2n, const double* var, const
     for (size_t e = 0; e < (size_t)n_edges; e++) {
         size_t a = (size_t)e2n[2 * e], b = (size_t)e2n[2 * e + 1];
         compute_flux_edge_kernel(var + 5 * a, var + 5 * b, ewt + 3 * e, flux + 5 * a, flux + 5 * b);
+    }
+}
+
+// Edge colouring for the multi-threaded CPU baseline: greedy, in edge order,
+// no two edges of one colour share a node (the scheme OP2's OpenMP backend
+// applies to its blocks).  order receives the edges grouped by colour,
+// cptr[c]..cptr[c+1] the range of colour c; returns the colour count (at
+// most max_colours, -1 if more are needed).
+int ref_colour_edges(int n_edges, int n_nodes, const int* e2n, int max_colours, int* order, int* cptr)
+{
+    std::vector<unsigned long long> used((size_t)n_nodes, 0ull);
+    std::vector<int> col((size_t)n_edges);
+    int nc = 0;
+    for (int e = 0; e < n_edges; e++) {
+        const int a = e2n[2 * (size_t)e], b = e2n[2 * (size_t)e + 1];
+        const unsigned long long u = used[a] | used[b];
+        if (~u == 0ull) return -1;
+        const int c = __builtin_ctzll(~u);
+        if (c >= max_colours) return -1;
+        col[e] = c;
+        used[a] |= 1ull << c;
+        used[b] |= 1ull << c;
+        if (c + 1 > nc) nc = c + 1;
+    }
+    std::vector<int> cnt((size_t)nc + 1, 0);
+    for (int e = 0; e < n_edges; e++) cnt[col[e] + 1]++;
+    for (int c = 0; c < nc; c++) cnt[c + 1] += cnt[c];
+    for (int c = 0; c <= nc; c++) cptr[c] = cnt[c];
+    for (int e = 0; e < n_edges; e++) order[cnt[col[e]]++] = e;
+    return nc;
+}
+
+// The reference's flux kernel over coloured edges, one OpenMP parallel loop
+// per colour: every node's increments still come one edge at a time, with no
+// two threads touching one node.
+void ref_compute_flux_edge_omp(int n_colours, const int* cptr, const int* order, const int* e2n,
+                               const double* var, const double* ewt, double* flux, const orc_consts* c,
+                               int threads)
+{
+    load_consts(c);
+    for (int k = 0; k < n_colours; k++) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int q = cptr[k]; q < cptr[k + 1]; q++) {
+            const size_t e = (size_t)order[q];
+            const size_t a = (size_t)e2n[2 * e], b = (size_t)e2n[2 * e + 1];
+            compute_flux_edge_kernel(var + 5 * a, var + 5 * b, ewt + 3 * e, flux + 5 * a, flux + 5 * b);
+        }
     }
 }
 
