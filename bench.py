@@ -51,12 +51,14 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "flux_traffic.json")
 COPY_BYTES = 512 << 20          # calibration copy: 512 MiB read + 512 MiB written per launch
 # demangled kernel name pattern of each edge-flux variant
 VARIANT_KERNELS = {
-    "gather_exact": r"k_flux_blocked<true>",
-    "gather_fast": r"k_flux_blocked<false>",
+    "gather_exact": r"k_flux_blocked_pf<true",
+    "gather_fast": r"k_flux_blocked_pf<false",
     "atomic": r"k_flux_atomic\(",
     "nodal_exact": r"k_flux_gather<true>",
-    "part_coloured": r"k_flux_part<\d+, \d+, true>",
-    "part_atomic": r"k_flux_part<\d+, \d+, false>",
+    "part_coloured": r"k_flux_part<\d+, \d+, true",
+    "part_atomic": r"k_flux_part<\d+, \d+, false",
+    "mixed_coloured": r"k_flux_part32<\d+, \d+, true",
+    "mixed_atomic": r"k_flux_part32<\d+, \d+, false",
 }
 
 

@@ -197,7 +197,11 @@ __device__ __forceinline__ bool halted(const int* status) { return status && *st
 //     each of its node's edges, reading the other endpoint from LDS.  The
 //     k-th incidences of all threads of a block are stored contiguously
 //     (jagged-diagonal order: jbase + jpref[k] + t), so the per-incidence
-//     loads {LDS slot | b-role << 31} and {w0, w1, w2, |w|} are coalesced.
+//     loads {LDS slot | b-role << 31} and the weight components w0 | w1 | w2
+//     (SoA) are coalesced.  They are double-buffered in registers: chunks
+//     of CH incidences, the first in flight with the staging loads, chunk
+//     c+1 loading while chunk c is evaluated.  |w| is recomputed with the
+//     reference's expression (flux.h:48-50).
 // Every node sums its edges in OP2's sequential order; with EXACT (no FMA
 // contraction, true divisions) the result is bitwise the reference's.
 // Deterministic, no atomics, one launch, one barrier.
@@ -226,11 +230,68 @@ __device__ __forceinline__ void load_record(const double* rec, Th& t)
     flux_tensor<EXACT>(t);
 }
 
-template <bool EXACT>
-__global__ __launch_bounds__(kBlock) void k_flux_blocked(
+// CH incidences of one thread: {slot | role} codes and weight components.
+template <int CH>
+struct IncChunk {
+    unsigned r[CH];
+    double x0[CH], x1[CH], x2[CH];
+};
+
+template <int CH>
+__device__ __forceinline__ void load_chunk(IncChunk<CH>& c, int q0, int deg, int jb, const int* __restrict__ jp,
+                                           const unsigned* __restrict__ jslot, const double* __restrict__ jw,
+                                           int nent)
+{
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+        const int q = q0 + i;
+        c.r[i] = 0;
+        c.x0[i] = c.x1[i] = c.x2[i] = 0.0;
+        if (q < deg) {
+            const int at = jb + jp[q];
+            c.r[i] = jslot[at];
+            c.x0[i] = jw[at];
+            c.x1[i] = jw[nent + at];
+            c.x2[i] = jw[2 * nent + at];
+        }
+    }
+}
+
+template <bool EXACT, int CH>
+__device__ __forceinline__ void walk_chunk(const IncChunk<CH>& c, int q0, int deg, const double* sm, const Th& o,
+                                           double sc, double acc[5])
+{
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+        if (q0 + i < deg) {
+            const unsigned r = c.r[i];
+            Th th;
+            load_record<EXACT>(sm + (int)(r & 0x7fffffffu) * kRec, th);
+            double ewt;
+            {
+#pragma clang fp contract(off)
+                ewt = sqrt(c.x0[i] * c.x0[i] + c.x1[i] * c.x1[i] + c.x2[i] * c.x2[i]);
+            }
+            double fx = -0.5 * c.x0[i], fy = -0.5 * c.x1[i], fz = -0.5 * c.x2[i];
+            if (r & 0x80000000u) { fx = -fx; fy = -fy; fz = -fz; }   // node is endpoint b
+            double d[5];
+            edge_increment<EXACT>(o, th, ewt, sc, fx, fy, fz, d);
+            if constexpr (EXACT) {
+#pragma unroll
+                for (int v = 0; v < 5; v++) acc[v] = add_exact(acc[v], d[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; v++) acc[v] += d[v];
+            }
+        }
+    }
+}
+
+template <bool EXACT, int CH>
+__global__ __launch_bounds__(kBlock) void k_flux_blocked_pf(
     const int2* __restrict__ bnode, const int* __restrict__ hptr, const int* __restrict__ halo,
     const int* __restrict__ jbase, const int* __restrict__ pbase, const int* __restrict__ jpref,
-    const int* __restrict__ nptr, const unsigned* __restrict__ jslot, const double4* __restrict__ jw,
+    const int* __restrict__ nptr, const unsigned* __restrict__ jslot, const double* __restrict__ jw, int nent,
     const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
 {
     extern __shared__ double sm[];
@@ -241,12 +302,16 @@ __global__ __launch_bounds__(kBlock) void k_flux_blocked(
     const int2 bn = bnode[b];
     const bool own = t < bn.y;
     const int nd = bn.x + t;
-    double uo[5], uh[5], acc[5];
     int deg = 0;
+    if (own) deg = nptr[nd + 1] - nptr[nd];
+    const int jb = jbase[b] + t;
+    const int* jp = jpref + pbase[b];
+    IncChunk<CH> A, B;
+    load_chunk<CH>(A, 0, deg, jb, jp, jslot, jw, nent);
+    double uo[5], uh[5], acc[5];
     if (own) {
         load5(var, nd, uo);
         load5(flux, nd, acc);
-        deg = nptr[nd + 1] - nptr[nd];
     }
     if (hn >= 0) load5(var, hn, uh);
     if (own) stage_record<EXACT>(sm + t * kRec, uo);
@@ -255,39 +320,28 @@ __global__ __launch_bounds__(kBlock) void k_flux_blocked(
     if (!own) return;
     Th o;
     load_record<EXACT>(sm + t * kRec, o);
-    const int jb = jbase[b] + t, pb = pbase[b];
-    for (int q = 0; q < deg; q++) {
-        const int at = jb + jpref[pb + q];
-        const unsigned r = jslot[at];
-        const double4 w = jw[at];
-        Th th;
-        load_record<EXACT>(sm + (int)(r & 0x7fffffffu) * kRec, th);
-        double fx = -0.5 * w.x, fy = -0.5 * w.y, fz = -0.5 * w.z;
-        if (r & 0x80000000u) { fx = -fx; fy = -fy; fz = -fz; }   // node is endpoint b
-        double d[5];
-        edge_increment<EXACT>(o, th, w.w, k.sc, fx, fy, fz, d);
-        if constexpr (EXACT) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) acc[v] = add_exact(acc[v], d[v]);
-        } else {
-#pragma unroll
-            for (int v = 0; v < 5; v++) acc[v] += d[v];
-        }
+    for (int q0 = 0; q0 < deg; q0 += 2 * CH) {
+        load_chunk<CH>(B, q0 + CH, deg, jb, jp, jslot, jw, nent);
+        walk_chunk<EXACT, CH>(A, q0, deg, sm, o, k.sc, acc);
+        if (q0 + CH >= deg) break;
+        load_chunk<CH>(A, q0 + 2 * CH, deg, jb, jp, jslot, jw, nent);
+        walk_chunk<EXACT, CH>(B, q0 + CH, deg, sm, o, k.sc, acc);
     }
     store5(flux, nd, acc);
 }
 
-// Per-incidence weights of the blocked gather, gathered from the weight dat.
-__global__ void k_pack_incidence_weights(int n, const int* __restrict__ eid, const double* __restrict__ w,
-                                         double4* __restrict__ out)
+// Per-incidence weight components (SoA) of k_flux_blocked_pf.
+__global__ void k_pack_incidence_weights3(int n, const int* __restrict__ eid, const double* __restrict__ w,
+                                          double* __restrict__ out)
 {
-#pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int e = eid[k];
-    if (e < 0) { out[k] = make_double4(0.0, 0.0, 0.0, 0.0); return; }
-    const double a = w[3 * (size_t)e], b = w[3 * (size_t)e + 1], c = w[3 * (size_t)e + 2];
-    out[k] = make_double4(a, b, c, sqrt(a * a + b * b + c * c));
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (e >= 0) { a = w[3 * (size_t)e]; b = w[3 * (size_t)e + 1]; c = w[3 * (size_t)e + 2]; }
+    out[k] = a;
+    out[n + k] = b;
+    out[2 * (size_t)n + k] = c;
 }
 
 // ------------------------------------------------------------------------

@@ -87,7 +87,7 @@ struct InvPlan {
     int* d_bk_jpref = nullptr;   // per block: offset of diagonal k (k = 0..max degree)
     unsigned* d_bk_jslot = nullptr;   // per entry: LDS slot of the other endpoint | b-role << 31
     int* d_bk_eid = nullptr;     // per entry: internal edge id (-1 = padding)
-    double4* d_bk_w = nullptr;   // per entry: packed weights
+    double* d_bk_w3 = nullptr;   // per entry: weight components w0 | w1 | w2 (SoA)
     const void* bk_wdat = nullptr;    // weight dat the packed weights came from
     unsigned bk_wver = 0;
     int bk_max_halo = 0;
@@ -516,7 +516,7 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
     TRY(up(P.d_bk_jpref, jpref));
     TRY(up(P.d_bk_jslot, jslot));
     TRY(up(P.d_bk_eid, eid));
-    HIPCHK(hipMalloc(&P.d_bk_w, sizeof(double4) * std::max<size_t>(1, eid.size())));
+    HIPCHK(hipMalloc(&P.d_bk_w3, sizeof(double) * 3 * std::max<size_t>(1, eid.size())));
     P.bk_blocks = (int)blocks.size();
     P.bk_entries = (int)eid.size();
     P.bk_max_halo = max_halo;
@@ -750,7 +750,7 @@ static void free_invplan(InvPlan& P)
 {
     for (void* p : {(void*)P.d_targets, (void*)P.d_ptr, (void*)P.d_inc2, (void*)P.d_inc1, (void*)P.d_bk_bnode,
                     (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase, (void*)P.d_bk_pbase,
-                    (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w})
+                    (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w3})
         if (p) hipFree(p);
     free_partplan(P);
     P = InvPlan();
@@ -870,8 +870,8 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
 {
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0 &&
         (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version)) {
-        k_pack_incidence_weights<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
-                                                                             P.d_bk_w);
+        k_pack_incidence_weights3<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
+                                                                              P.d_bk_w3);
         TRY(check_launch());
         P.bk_wdat = w;
         P.bk_wver = D(w)->version;
@@ -906,6 +906,19 @@ static int launch_part_t(bool coloured, const InvPlan& P, op_dat var, op_dat flu
         k_flux_part<T, EPT, false><<<P.pt_parts, T, P.pt_lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
             status);
+    return check_launch();
+}
+
+static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+{
+    if (exact)
+        k_flux_blocked_pf<true, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
+            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status);
+    else
+        k_flux_blocked_pf<false, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
+            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status);
     return check_launch();
 }
 
@@ -976,17 +989,8 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
         if (mixed_variant(variant)) return launch_part32(variant == MGCFD_FLUX_MIXED_COLOURED, P, var, flux, status);
         return launch_part(variant == MGCFD_FLUX_PART_COLOURED, P, var, flux, status);
     }
-    if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0) {
-        if (variant == MGCFD_FLUX_GATHER_EXACT)
-            k_flux_blocked<true><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
-                P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr,
-                P.d_bk_jslot, P.d_bk_w, dptr(var), dptr(flux), R.k, status);
-        else
-            k_flux_blocked<false><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
-                P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr,
-                P.d_bk_jslot, P.d_bk_w, dptr(var), dptr(flux), R.k, status);
-        return check_launch();
-    }
+    if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0)
+        return launch_blocked_pf(variant == MGCFD_FLUX_GATHER_EXACT, P, var, flux, status);
     const double4* wp = nullptr;
     TRY(packed_weights(w, &wp));
     if (variant == MGCFD_FLUX_GATHER_FAST)
