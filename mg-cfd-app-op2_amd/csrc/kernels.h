@@ -349,24 +349,25 @@ __global__ void k_pack_incidence_weights3(int n, const int* __restrict__ eid, co
 // in LDS.  Workgroup b owns the nodes [n0, n0 + cnt) of one compact part and
 // every edge with at least one endpoint among them ("local edges": interior
 // edges once, cut edges once on each side).
-//  1. Load (all loads issued before the first barrier, flat and coalesced):
-//     owned variables, the old flux of owned nodes (into the LDS
-//     accumulators), halo variables, and this thread's local edges {slot a,
-//     slot b, owner flags, colour} with their weights.
-//  2. Each staged node's record {rho, m, E, u, p, |u| + c} is completed in
-//     LDS (one density reciprocal, fast form).
-//  3. Each thread evaluates its edges once: inc = factor (U_a - U_b) +
-//     sum_k f_k (F_a + F_b)_k with f = -w/2; node a gains +inc and node b
-//     gains -inc (flux.h:139-207 with factor_b == factor_a up to rounding).
-//  4. Accumulation into the LDS accumulators: COLOURED walks the edge
+//  1. Load (all loads issued before the barrier, flat and coalesced): owned
+//     variables and halo variables as raw {rho, m, E} records in LDS, the
+//     old flux of owned nodes into the LDS accumulators, and this thread's
+//     local edges {slot a, slot b, owner flags, colour} with their weights.
+//  2. Each thread evaluates its edges once: both endpoint states from their
+//     LDS records (one density reciprocal each, fast form), then inc =
+//     factor (U_a - U_b) + sum_k f_k (F_a + F_b)_k with f = -w/2; node a
+//     gains +inc and node b gains -inc (flux.h:139-207 with factor_b ==
+//     factor_a up to rounding).  Raw records keep the LDS footprint at
+//     48 B per staged node, so more workgroups share a CU.
+//  3. Accumulation into the LDS accumulators: COLOURED walks the edge
 //     colours of the part (no two edges of one colour share an owned node),
 //     one barrier per colour, so the summation order is fixed by the plan and
 //     results are deterministic; otherwise LDS FP64 atomics (ds_add_f64) in
 //     arrival order.
-//  5. The accumulators are stored as one coalesced block.
+//  4. The accumulators are stored as one coalesced block.
 // Not bitwise the sequential order (ulp-level drift, as the fast gather).
 // ------------------------------------------------------------------------
-constexpr int kPRec = 10;   // doubles per LDS node record of k_flux_part
+constexpr int kPRec = 6;    // doubles per LDS node record of k_flux_part {rho, mx, my, mz, E, -}
 
 struct PartInfo {
     int n0, cnt;    // owned nodes [n0, n0 + cnt)
@@ -375,83 +376,144 @@ struct PartInfo {
     int ncol, pad;  // edge colours (COLOURED)
 };
 
-__device__ __forceinline__ void part_record(double* r)
+template <typename R>
+struct PState {
+    R rho, mx, my, mz, E, ux, uy, uz, p, s;   // s = |u| + c
+};
+
+__device__ __forceinline__ PState<double> part_state(const double* r)
 {
-    const double rho = r[0], mx = r[1], my = r[2], mz = r[3], E = r[4];
-    const double inv = 1.0 / rho;
-    const double ux = mx * inv, uy = my * inv, uz = mz * inv;
-    const double ssq = ux * ux + uy * uy + uz * uz;
-    const double p = (kGamma - 1.0) * (E - 0.5 * rho * ssq);
-    const double c = sqrt(kGamma * p * inv);
-    r[5] = ux; r[6] = uy; r[7] = uz; r[8] = p; r[9] = sqrt(ssq) + c;
+    const double2* q = reinterpret_cast<const double2*>(r);
+    const double2 a = q[0], b = q[1], c = q[2];
+    PState<double> t;
+    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
+    const double inv = 1.0 / t.rho;
+    t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
+    const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
+    t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
+    t.s = sqrt(ssq) + sqrt(kGamma * t.p * inv);
+    return t;
 }
 
-// Increment of node a for edge (a, b) in the fast form; node b gains -inc.
-__device__ __forceinline__ void part_increment(const double* ra, const double* rb, double w0, double w1,
-                                               double w2, double sc, double inc[5])
+__device__ __forceinline__ PState<float> part_state32(const float* r)
 {
-    const double2* A = reinterpret_cast<const double2*>(ra);
-    const double2* B = reinterpret_cast<const double2*>(rb);
-    const double2 a0 = A[0], a1 = A[1], a2 = A[2], a3 = A[3], a4 = A[4];
-    const double2 b0 = B[0], b1 = B[1], b2 = B[2], b3 = B[3], b4 = B[4];
-    // a: rho a0.x, mx a0.y, my a1.x, mz a1.y, E a2.x, ux a2.y, uy a3.x, uz a3.y, p a4.x, s a4.y
-    const double ewt = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
-    const double factor = -ewt * sc * 0.5 * (a4.y + b4.y);
-    const double fx = -0.5 * w0, fy = -0.5 * w1, fz = -0.5 * w2;
-    const double pa = a4.x, pb = b4.x;
-    const double dea = a2.x + pa, deb = b2.x + pb;
-    // momentum flux tensors F_a + F_b (symmetric)
-    const double sxx = a2.y * a0.y + pa + b2.y * b0.y + pb;
-    const double sxy = a2.y * a1.x + b2.y * b1.x;
-    const double sxz = a2.y * a1.y + b2.y * b1.y;
-    const double syy = a3.x * a1.x + pa + b3.x * b1.x + pb;
-    const double syz = a3.x * a1.y + b3.x * b1.y;
-    const double szz = a3.y * a1.y + pa + b3.y * b1.y + pb;
-    const double sdx = a2.y * dea + b2.y * deb;
-    const double sdy = a3.x * dea + b3.x * deb;
-    const double sdz = a3.y * dea + b3.y * deb;
-    inc[0] = factor * (a0.x - b0.x) + fx * (a0.y + b0.y) + fy * (a1.x + b1.x) + fz * (a1.y + b1.y);
-    inc[1] = factor * (a0.y - b0.y) + fx * sxx + fy * sxy + fz * sxz;
-    inc[2] = factor * (a1.x - b1.x) + fx * sxy + fy * syy + fz * syz;
-    inc[3] = factor * (a1.y - b1.y) + fx * sxz + fy * syz + fz * szz;
-    inc[4] = factor * (a2.x - b2.x) + fx * sdx + fy * sdy + fz * sdz;
+    const float2* q = reinterpret_cast<const float2*>(r);
+    const float2 a = q[0], b = q[1], c = q[2];
+    PState<float> t;
+    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
+    const float inv = 1.0f / t.rho;
+    t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
+    const float ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
+    t.p = (float)(kGamma - 1.0) * (t.E - 0.5f * t.rho * ssq);
+    t.s = sqrtf(ssq) + sqrtf((float)kGamma * t.p * inv);
+    return t;
 }
 
-__device__ __forceinline__ void part_apply(double* acc, unsigned code, const double inc[5])
+// Increment of node a for edge (a, b) in the fast form (FP64 or FP32
+// arithmetic); node b gains -inc.
+template <typename R, typename O>
+__device__ __forceinline__ void part_increment(const PState<R>& a, const PState<R>& b, R w0, R w1, R w2, R sc,
+                                               O inc[5])
+{
+    const R ewt = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const R factor = -ewt * sc * R(0.5) * (a.s + b.s);
+    const R fx = R(-0.5) * w0, fy = R(-0.5) * w1, fz = R(-0.5) * w2;
+    const R dea = a.E + a.p, deb = b.E + b.p;
+    // momentum flux tensors F_a + F_b (symmetric) and energy flux
+    const R sxx = a.ux * a.mx + a.p + b.ux * b.mx + b.p;
+    const R sxy = a.ux * a.my + b.ux * b.my;
+    const R sxz = a.ux * a.mz + b.ux * b.mz;
+    const R syy = a.uy * a.my + a.p + b.uy * b.my + b.p;
+    const R syz = a.uy * a.mz + b.uy * b.mz;
+    const R szz = a.uz * a.mz + a.p + b.uz * b.mz + b.p;
+    const R sdx = a.ux * dea + b.ux * deb;
+    const R sdy = a.uy * dea + b.uy * deb;
+    const R sdz = a.uz * dea + b.uz * deb;
+    inc[0] = (O)(factor * (a.rho - b.rho) + fx * (a.mx + b.mx) + fy * (a.my + b.my) + fz * (a.mz + b.mz));
+    inc[1] = (O)(factor * (a.mx - b.mx) + fx * sxx + fy * sxy + fz * sxz);
+    inc[2] = (O)(factor * (a.my - b.my) + fx * sxy + fy * syy + fz * syz);
+    inc[3] = (O)(factor * (a.mz - b.mz) + fx * sxz + fy * syz + fz * szz);
+    inc[4] = (O)(factor * (a.E - b.E) + fx * sdx + fy * sdy + fz * sdz);
+}
+
+// Adds inc to owned endpoint a and -inc to owned endpoint b (FP64 adds; an
+// FP32 increment is converted first).
+template <typename O>
+__device__ __forceinline__ void part_apply(double* acc, unsigned code, const O inc[5])
 {
     const int sa = code & 1023u, sb = (code >> 10) & 1023u;
     if (code & (1u << 20)) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) atomicAdd(acc + sa * 5 + v, inc[v]);
+        for (int v = 0; v < 5; v++) atomicAdd(acc + sa * 5 + v, (double)inc[v]);
     }
     if (code & (1u << 21)) {
 #pragma unroll
-        for (int v = 0; v < 5; v++) atomicAdd(acc + sb * 5 + v, -inc[v]);
+        for (int v = 0; v < 5; v++) atomicAdd(acc + sb * 5 + v, -(double)inc[v]);
     }
 }
 
-// LDS: records [(cnt + nh) * kPRec] doubles, then accumulators [cnt * 5].
-// The launch passes rec_cap = max over parts of (cnt + nh).
-template <int T, int EPT, bool COLOURED>
+// LDS node records of the part kernels.  FP64: the raw state {rho, m, E}
+// (6 doubles), both endpoint states derived per edge -- the smaller record
+// lets more workgroups share a CU.  FP32: the derived state {rho, m, E, u,
+// p, |u| + c} (12 floats) is built once per staged node after the loads --
+// cheaper than deriving it per edge and still small.
+template <typename R>
+struct PartRec;
+template <>
+struct PartRec<double> {
+    static constexpr int kStride = 6;
+    static constexpr bool kDerived = false;
+    static __device__ __forceinline__ PState<double> state(const double* r) { return part_state(r); }
+    static __device__ __forceinline__ void derive(double*) {}
+};
+template <>
+struct PartRec<float> {
+    static constexpr int kStride = 12;
+    static constexpr bool kDerived = true;
+    static __device__ __forceinline__ PState<float> state(const float* r)
+    {
+        const float4* q = reinterpret_cast<const float4*>(r);
+        const float4 a = q[0], b = q[1], c = q[2];
+        PState<float> t;
+        t.rho = a.x; t.mx = a.y; t.my = a.z; t.mz = a.w;
+        t.E = b.x; t.ux = b.y; t.uy = b.z; t.uz = b.w;
+        t.p = c.x; t.s = c.y;
+        return t;
+    }
+    static __device__ __forceinline__ void derive(float* r)
+    {
+        const PState<float> t = part_state32(r);
+        float4* w = reinterpret_cast<float4*>(r);
+        w[1] = make_float4(t.E, t.ux, t.uy, t.uz);
+        w[2] = make_float4(t.p, t.s, 0.f, 0.f);
+    }
+};
+
+// LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
+// accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
+// precision (records, weights and increment arithmetic in FP32, each
+// increment converted to FP64 and accumulated in FP64 from the FP64 flux).
+template <typename R, int T, int EPT, bool COLOURED>
 __global__ __launch_bounds__(T) void k_flux_part(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
-    const double* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
     double* __restrict__ flux, Consts k, const int* __restrict__ status)
 {
+    constexpr int RS = PartRec<R>::kStride;
     extern __shared__ double sm[];
     if (halted(status)) return;
     const PartInfo P = parts[xcd_block()];
     const int t = threadIdx.x;
-    double* rec = sm;
-    double* acc = sm + (size_t)rec_cap * kPRec;
+    R* rec = reinterpret_cast<R*>(sm);
+    double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
     // 1. loads
     unsigned code[EPT];
-    double w0[EPT], w1[EPT], w2[EPT];
+    R w0[EPT], w1[EPT], w2[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const int j = t + i * T;
         code[i] = 0;
-        w0[i] = w1[i] = w2[i] = 0.0;
+        w0[i] = w1[i] = w2[i] = R(0);
         if (j < P.ne) {
             const long long e = P.e0 + j;
             code[i] = ecode[e];
@@ -464,24 +526,29 @@ __global__ __launch_bounds__(T) void k_flux_part(
     const double* fo = flux + (size_t)P.n0 * 5;
     for (int j = t; j < P.cnt * 5; j += T) {
         const int nd = j / 5, v = j - nd * 5;
-        rec[nd * kPRec + v] = vo[j];
+        rec[nd * RS + v] = (R)vo[j];
         acc[j] = fo[j];
     }
     for (int j = t; j < P.nh * 5; j += T) {
         const int h = j / 5, v = j - h * 5;
-        rec[(P.cnt + h) * kPRec + v] = var[(size_t)halo[P.h0 + h] * 5 + v];
+        rec[(P.cnt + h) * RS + v] = (R)var[(size_t)halo[P.h0 + h] * 5 + v];
     }
     __syncthreads();
-    // 2. records
-    for (int s = t; s < P.cnt + P.nh; s += T) part_record(rec + s * kPRec);
-    __syncthreads();
-    // 3./4. edges
+    if constexpr (PartRec<R>::kDerived) {
+        for (int s2 = t; s2 < P.cnt + P.nh; s2 += T) PartRec<R>::derive(rec + s2 * RS);
+        __syncthreads();
+    }
+    // 2./3. edges
+    const R sc = (R)k.sc;
     if constexpr (COLOURED) {
-        double inc[EPT][5];
+        R inc[EPT][5];      // held in the arithmetic's precision until the colour walk
 #pragma unroll
         for (int i = 0; i < EPT; i++)
-            if (code[i]) part_increment(rec + (code[i] & 1023u) * kPRec, rec + ((code[i] >> 10) & 1023u) * kPRec,
-                                        w0[i], w1[i], w2[i], k.sc, inc[i]);
+            if (code[i]) {
+                const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
+                const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
+                part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc[i]);
+            }
         for (int c = 0; c < P.ncol; c++) {
 #pragma unroll
             for (int i = 0; i < EPT; i++)
@@ -492,138 +559,20 @@ __global__ __launch_bounds__(T) void k_flux_part(
 #pragma unroll
         for (int i = 0; i < EPT; i++) {
             if (!code[i]) continue;
-            double inc[5];
-            part_increment(rec + (code[i] & 1023u) * kPRec, rec + ((code[i] >> 10) & 1023u) * kPRec, w0[i], w1[i],
-                           w2[i], k.sc, inc);
+            const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
+            const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
+            R inc[5];
+            part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc);
             part_apply(acc, code[i], inc);
         }
         __syncthreads();
     }
-    // 5. store
+    // 4. store
     double* fw = flux + (size_t)P.n0 * 5;
     for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
 }
 
-// ------------------------------------------------------------------------
-// Edge flux, mixed precision: the part kernel above with the flux
-// arithmetic in FP32 (node records, weights and increments) and FP64
-// accumulation (the LDS accumulators start from the FP64 flux and receive
-// each FP32 increment converted to FP64).  Records are 12 floats:
-// {rho, mx, my, mz | E, ux, uy, uz | p, |u| + c, -, -}.
-// ------------------------------------------------------------------------
-constexpr int kPRec32 = 12;
-
-__device__ __forceinline__ void part_record32(float* r, const double* U)
-{
-    const float rho = (float)U[0], mx = (float)U[1], my = (float)U[2], mz = (float)U[3], E = (float)U[4];
-    const float inv = 1.0f / rho;
-    const float ux = mx * inv, uy = my * inv, uz = mz * inv;
-    const float ssq = ux * ux + uy * uy + uz * uz;
-    const float p = (float)(kGamma - 1.0) * (E - 0.5f * rho * ssq);
-    const float c = sqrtf((float)kGamma * p * inv);
-    float4* R4 = reinterpret_cast<float4*>(r);
-    R4[0] = make_float4(rho, mx, my, mz);
-    R4[1] = make_float4(E, ux, uy, uz);
-    R4[2] = make_float4(p, sqrtf(ssq) + c, 0.f, 0.f);
-}
-
-__device__ __forceinline__ void part_increment32(const float* ra, const float* rb, float w0, float w1, float w2,
-                                                 float sc, double inc[5])
-{
-    const float4* A = reinterpret_cast<const float4*>(ra);
-    const float4* B = reinterpret_cast<const float4*>(rb);
-    const float4 a0 = A[0], a1 = A[1], a2 = A[2], b0 = B[0], b1 = B[1], b2 = B[2];
-    // a0 {rho, mx, my, mz}, a1 {E, ux, uy, uz}, a2 {p, s}
-    const float ewt = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
-    const float factor = -ewt * sc * 0.5f * (a2.y + b2.y);
-    const float fx = -0.5f * w0, fy = -0.5f * w1, fz = -0.5f * w2;
-    const float pa = a2.x, pb = b2.x;
-    const float dea = a1.x + pa, deb = b1.x + pb;
-    const float sxx = a1.y * a0.y + pa + b1.y * b0.y + pb;
-    const float sxy = a1.y * a0.z + b1.y * b0.z;
-    const float sxz = a1.y * a0.w + b1.y * b0.w;
-    const float syy = a1.z * a0.z + pa + b1.z * b0.z + pb;
-    const float syz = a1.z * a0.w + b1.z * b0.w;
-    const float szz = a1.w * a0.w + pa + b1.w * b0.w + pb;
-    const float sdx = a1.y * dea + b1.y * deb;
-    const float sdy = a1.z * dea + b1.z * deb;
-    const float sdz = a1.w * dea + b1.w * deb;
-    inc[0] = (double)(factor * (a0.x - b0.x) + fx * (a0.y + b0.y) + fy * (a0.z + b0.z) + fz * (a0.w + b0.w));
-    inc[1] = (double)(factor * (a0.y - b0.y) + fx * sxx + fy * sxy + fz * sxz);
-    inc[2] = (double)(factor * (a0.z - b0.z) + fx * sxy + fy * syy + fz * syz);
-    inc[3] = (double)(factor * (a0.w - b0.w) + fx * sxz + fy * syz + fz * szz);
-    inc[4] = (double)(factor * (a1.x - b1.x) + fx * sdx + fy * sdy + fz * sdz);
-}
-
-// LDS: FP32 records [(cnt + nh) * kPRec32] floats (rounded up to doubles),
-// then FP64 accumulators [cnt * 5].
-template <int T, int EPT, bool COLOURED>
-__global__ __launch_bounds__(T) void k_flux_part32(
-    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
-    const float* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
-    double* __restrict__ flux, Consts k, const int* __restrict__ status)
-{
-    extern __shared__ double sm[];
-    if (halted(status)) return;
-    const PartInfo P = parts[xcd_block()];
-    const int t = threadIdx.x;
-    float* rec = reinterpret_cast<float*>(sm);
-    double* acc = sm + ((size_t)rec_cap * kPRec32 + 1) / 2;
-    unsigned code[EPT];
-    float w0[EPT], w1[EPT], w2[EPT];
-#pragma unroll
-    for (int i = 0; i < EPT; i++) {
-        const int j = t + i * T;
-        code[i] = 0;
-        w0[i] = w1[i] = w2[i] = 0.f;
-        if (j < P.ne) {
-            const long long e = P.e0 + j;
-            code[i] = ecode[e];
-            w0[i] = ew[e];
-            w1[i] = ew[nloc + e];
-            w2[i] = ew[2 * nloc + e];
-        }
-    }
-    // node records straight from the FP64 state, one node per thread
-    const double* fo = flux + (size_t)P.n0 * 5;
-    for (int j = t; j < P.cnt * 5; j += T) acc[j] = fo[j];
-    for (int s2 = t; s2 < P.cnt + P.nh; s2 += T) {
-        const int nd = s2 < P.cnt ? P.n0 + s2 : halo[P.h0 + s2 - P.cnt];
-        double U[5];
-        load5(var, nd, U);
-        part_record32(rec + s2 * kPRec32, U);
-    }
-    __syncthreads();
-    const float sc = (float)k.sc;
-    if constexpr (COLOURED) {
-        double inc[EPT][5];
-#pragma unroll
-        for (int i = 0; i < EPT; i++)
-            if (code[i])
-                part_increment32(rec + (code[i] & 1023u) * kPRec32, rec + ((code[i] >> 10) & 1023u) * kPRec32,
-                                 w0[i], w1[i], w2[i], sc, inc[i]);
-        for (int c = 0; c < P.ncol; c++) {
-#pragma unroll
-            for (int i = 0; i < EPT; i++)
-                if (code[i] && (int)(code[i] >> 22) == c) part_apply(acc, code[i], inc[i]);
-            __syncthreads();
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < EPT; i++) {
-            if (!code[i]) continue;
-            double inc[5];
-            part_increment32(rec + (code[i] & 1023u) * kPRec32, rec + ((code[i] >> 10) & 1023u) * kPRec32, w0[i],
-                             w1[i], w2[i], sc, inc);
-            part_apply(acc, code[i], inc);
-        }
-        __syncthreads();
-    }
-    double* fw = flux + (size_t)P.n0 * 5;
-    for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
-}
-
-// FP32 local-edge weights of k_flux_part32.
+// Local-edge weights of the mixed-precision part kernel, in FP32.
 __global__ void k_pack_part_weights32(long long n, const int* __restrict__ eid, const double* __restrict__ w,
                                       float* __restrict__ out)
 {

@@ -300,7 +300,23 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
 }
 
 static int upload_dat(op_dat d);
-static void mg_release_all();     // multigrid contexts and batches (mgrun.inc)
+static void mg_release_all();
+
+// Host set-up timing (MGCFD_TIMING=1 in the environment): one line per
+// partition / plan build on stderr.
+struct HostTimer {
+    const char* what;
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    HostTimer(const char* w, const char* n) : what(w), name(n ? n : ""), t0(std::chrono::steady_clock::now()) {}
+    ~HostTimer()
+    {
+        static const bool on = std::getenv("MGCFD_TIMING") != nullptr;
+        if (!on) return;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "mgcfd timing: %s %s %.1f ms\n", what, name, ms);
+    }
+};     // multigrid contexts and batches (mgrun.inc)
 
 // Part boundaries of a node set's internal order (kPart-node chunks when the
 // set was not partitioned).
@@ -346,7 +362,10 @@ static int freeze()
             if (m.impl && m.to == &s && m.dim == 2 && m.from != &s) { ems.push_back(M(&m)); fsz.push_back(m.from->size); }
         if (ems.empty() || s.size == 0) continue;
         SetImpl* si = S(&s);
-        si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts);
+        {
+            HostTimer tm("partition", s.name);
+            si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts);
+        }
         si->iperm.assign(s.size, -1);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
         for (int i = 0; i < s.size; i++)
@@ -649,7 +668,8 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     P.pt_max_ne = max_ne;
     P.pt_max_col = max_col;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
-    P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * kPRec32 + 1) / 2 + (size_t)kPart * 5);
+    P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * PartRec<float>::kStride * sizeof(float) + 7) / 8 +
+                                   (size_t)kPart * 5);
     return 0;
 }
 
@@ -659,6 +679,7 @@ static int build_inverse(op_map m)
     InvPlan& P = mi->inv;
     if (P.built) return 0;
     if (int rc = freeze()) return rc;
+    HostTimer tm("plans", m->name);
     SetImpl* fs = S(m->from);
     SetImpl* ts = S(m->to);
     const int nf = m->from->size, nt = m->to->size, dim = m->dim;
@@ -895,20 +916,6 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
     return 0;
 }
 
-template <int T, int EPT>
-static int launch_part_t(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
-{
-    if (coloured)
-        k_flux_part<T, EPT, true><<<P.pt_parts, T, P.pt_lds, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
-            status);
-    else
-        k_flux_part<T, EPT, false><<<P.pt_parts, T, P.pt_lds, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k,
-            status);
-    return check_launch();
-}
-
 static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat flux, const int* status)
 {
     if (exact)
@@ -922,37 +929,29 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
     return check_launch();
 }
 
-template <int T, int EPT>
-static int launch_part32_t(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+template <typename Real, int EPT>
+static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                         const int* status)
 {
     if (coloured)
-        k_flux_part32<T, EPT, true><<<P.pt_parts, T, P.pt_lds32, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w32, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux),
-            R.k, status);
+        k_flux_part<Real, kBlock, EPT, true><<<P.pt_parts, kBlock, lds, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     else
-        k_flux_part32<T, EPT, false><<<P.pt_parts, T, P.pt_lds32, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, P.d_pt_w32, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux),
-            R.k, status);
+        k_flux_part<Real, kBlock, EPT, false><<<P.pt_parts, kBlock, lds, R.stream>>>(
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     return check_launch();
 }
 
-static int launch_part32(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+// Real = double: the FP64 part kernel; Real = float: mixed precision.
+template <typename Real>
+static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                       const int* status)
 {
     const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
-    if (ept <= 4) return launch_part32_t<kBlock, 4>(coloured, P, var, flux, status);
-    if (ept <= 6) return launch_part32_t<kBlock, 6>(coloured, P, var, flux, status);
-    if (ept <= 8) return launch_part32_t<kBlock, 8>(coloured, P, var, flux, status);
-    if (ept <= 12) return launch_part32_t<kBlock, 12>(coloured, P, var, flux, status);
-    return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
-}
-
-static int launch_part(bool coloured, const InvPlan& P, op_dat var, op_dat flux, const int* status)
-{
-    const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
-    if (ept <= 4) return launch_part_t<kBlock, 4>(coloured, P, var, flux, status);
-    if (ept <= 6) return launch_part_t<kBlock, 6>(coloured, P, var, flux, status);
-    if (ept <= 8) return launch_part_t<kBlock, 8>(coloured, P, var, flux, status);
-    if (ept <= 12) return launch_part_t<kBlock, 12>(coloured, P, var, flux, status);
+    if (ept <= 4) return launch_part_t<Real, 4>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 6) return launch_part_t<Real, 6>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 8) return launch_part_t<Real, 8>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 12) return launch_part_t<Real, 12>(coloured, P, w, lds, var, flux, status);
     return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
 }
 
@@ -986,8 +985,10 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     TRY(refresh_plan_weights(variant, P, w));
     if (part_variant(variant) || mixed_variant(variant)) {
         if (P.pt_parts == 0) return fail("edge-flux variant %d needs an edge map onto every node of its set", variant);
-        if (mixed_variant(variant)) return launch_part32(variant == MGCFD_FLUX_MIXED_COLOURED, P, var, flux, status);
-        return launch_part(variant == MGCFD_FLUX_PART_COLOURED, P, var, flux, status);
+        if (mixed_variant(variant))
+            return launch_part<float>(variant == MGCFD_FLUX_MIXED_COLOURED, P, P.d_pt_w32, P.pt_lds32, var, flux,
+                                      status);
+        return launch_part<double>(variant == MGCFD_FLUX_PART_COLOURED, P, P.d_pt_w, P.pt_lds, var, flux, status);
     }
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0)
         return launch_blocked_pf(variant == MGCFD_FLUX_GATHER_EXACT, P, var, flux, status);
