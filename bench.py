@@ -261,7 +261,9 @@ def bench_serving(rt, n_req, rate, seed=50):
     meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=2000 + i) for i, n in enumerate(sizes)]
     log(f"bench: serving: generated {n_req} meshes ({int(sizes.min())}..{int(sizes.max())} nodes), "
         f"{time.perf_counter() - t:.1f} s")
-    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=10)
+    # every request's 200 cycles are issued at admission (one stream of graph
+    # replays); the convergence stop halts the rest on the device
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=200)
     ids, done = [], {}
     t0 = time.perf_counter()
     k = 0

@@ -117,11 +117,14 @@ int  op_get_size(op_set set);                  /* euler3d.cpp:537,700,728-729 */
 int  op_fetch_data_char(op_dat dat, char* dest);
 /* Host -> device overwrite of a dat (AoS, original order). */
 int  op_upload_data_char(op_dat dat, const char* src);
-/* op_partition / op_renumber (euler3d.cpp:340-377): on one GPU both become
- * the locality renumbering of every node set (recursive graph bisection over
- * its edge maps into compact parts of 256 nodes, one per flux workgroup) and
- * the matching reorder of edge and boundary sets.  Results are unaffected;
- * op_fetch_data always returns the original order. */
+/* op_partition / op_renumber (euler3d.cpp:340-377): one GPU has nothing to
+ * distribute.  Every node set gets a locality order at the first loop --
+ * compact parts of 256 nodes, one per flux workgroup -- and edge and
+ * boundary sets follow their nodes.  The order is geometric (Z-order of the
+ * coordinates) for a set passed to op_partition with a dim-3 coordinate dat
+ * before its first loop, else a recursive graph bisection over its edge
+ * maps.  Results are unaffected; op_fetch_data always returns the original
+ * order. */
 int  op_partition(const char* lib, const char* method, op_set set, op_map map, op_dat coords);
 int  op_renumber(op_map map);
 void op_timers(double* cpu, double* wall);     /* euler3d.cpp:450,645 */

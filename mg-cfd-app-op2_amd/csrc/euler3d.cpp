@@ -395,7 +395,8 @@ int main(int argc, char** argv)
     }
     op_printf("-----------------------------------------------------\n");
     op_printf("Partitioning ...\n");
-    op_check_(op_partition("PARMETIS", "KWAY", nodes[0], e2n[0], xyz[0]), "op_partition");
+    for (int i = 0; i < levels; i++)
+        op_check_(op_partition("PARMETIS", "KWAY", nodes[i], e2n[i], xyz[i]), "op_partition");
     op_printf("PARTITIONING COMPLETE\n");
 
     for (int i = 0; i < levels; i++) {

@@ -59,6 +59,7 @@ struct SetImpl {
     std::vector<int> iperm;      // original -> internal (empty: identity)
     std::vector<int> seg;        // segment offsets (batched meshes), empty: one segment
     std::vector<int> parts;      // part boundaries of the node order (size parts + 1)
+    std::vector<double> geo;     // node coordinates [n][3] (original order) when known: geometric order
     bool frozen = false;         // internal order chosen
     int to_int(int orig) const { return iperm.empty() ? orig : iperm[orig]; }
     int to_orig(int in) const { return perm.empty() ? in : perm[in]; }
@@ -208,9 +209,36 @@ static int check_launch()
 // pseudo-peripheral node and cuts it at a multiple of kPart, so every part
 // is a compact, mostly connected patch of the mesh: the workgroup that owns
 // a part finds most neighbours of its nodes inside the part.
+// Morton (Z-order) key of each node of [lo, hi): coordinates quantised to
+// 21 bits per axis over the range's bounding box, bits interleaved.
+static void morton_order(const std::vector<double>& geo, int lo, int hi, std::vector<int>& order)
+{
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int i = lo; i < hi; i++)
+        for (int a = 0; a < 3; a++) {
+            mn[a] = std::min(mn[a], geo[3 * (size_t)i + a]);
+            mx[a] = std::max(mx[a], geo[3 * (size_t)i + a]);
+        }
+    std::vector<std::pair<uint64_t, int>> key(hi - lo);
+    for (int i = lo; i < hi; i++) {
+        uint64_t q[3];
+        for (int a = 0; a < 3; a++) {
+            const double span = mx[a] - mn[a];
+            const double f = span > 0 ? (geo[3 * (size_t)i + a] - mn[a]) / span : 0.0;
+            q[a] = (uint64_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
+        }
+        uint64_t k = 0;
+        for (int b = 20; b >= 0; b--)
+            for (int a = 0; a < 3; a++) k = (k << 1) | ((q[a] >> b) & 1u);
+        key[i - lo] = {k, i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = lo; i < hi; i++) order[i] = key[i - lo].second;
+}
+
 static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>& emaps,
                                         const std::vector<int>& from_sizes, const std::vector<int>& segs,
-                                        std::vector<int>& parts)
+                                        std::vector<int>& parts, const std::vector<double>& geo)
 {
     std::vector<int> ptr(n + 1, 0);
     for (size_t m = 0; m < emaps.size(); m++) {
@@ -254,9 +282,15 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
     // one independent bisection per segment (batched meshes never share a part)
     std::vector<int> so = segs.empty() ? std::vector<int>{0, n} : segs;
     std::vector<Range> stack;
-    for (size_t k = so.size() - 1; k-- > 0;) {
-        for (int i = so[k]; i < so[k + 1]; i++) seg[i] = next_seg;
-        stack.push_back({so[k], so[k + 1], next_seg++});
+    if (!geo.empty()) {
+        // coordinates known (op_partition with a coordinate dat, batched and
+        // served meshes): Z-order per segment, cut into kPart-node parts
+        for (size_t k = 0; k + 1 < so.size(); k++) morton_order(geo, so[k], so[k + 1], order);
+    } else {
+        for (size_t k = so.size() - 1; k-- > 0;) {
+            for (int i = so[k]; i < so[k + 1]; i++) seg[i] = next_seg;
+            stack.push_back({so[k], so[k + 1], next_seg++});
+        }
     }
     std::vector<int> buf;
     while (!stack.empty()) {
@@ -301,6 +335,14 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
 
 static int upload_dat(op_dat d);
 static void mg_release_all();
+
+// Node coordinates [n][3] of a set not yet ordered: its internal order will
+// be geometric.
+static void set_geometry(op_set s, const double* xyz)
+{
+    SetImpl* si = static_cast<SetImpl*>(s->impl);
+    si->geo.assign(xyz, xyz + 3 * (size_t)s->size);
+}
 
 // Host set-up timing (MGCFD_TIMING=1 in the environment): one line per
 // partition / plan build on stderr.
@@ -364,7 +406,9 @@ static int freeze()
         SetImpl* si = S(&s);
         {
             HostTimer tm("partition", s.name);
-            si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts);
+            si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts, si->geo);
+            si->geo.clear();
+            si->geo.shrink_to_fit();
         }
         si->iperm.assign(s.size, -1);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
@@ -1257,10 +1301,18 @@ int op_upload_data_char(op_dat d, const char* src)
     return upload_dat(d);
 }
 
+// One GPU: nothing to distribute.  A coordinate dat on the set (as
+// euler3d.cpp:340-377 passes) selects the geometric (Z-order) internal node
+// order for that set instead of the graph bisection; the order itself is
+// chosen at the first loop, as for every set.
 int op_partition(const char* lib, const char* method, op_set set, op_map map, op_dat coords)
 {
-    (void)lib; (void)method; (void)set; (void)map; (void)coords;
-    return freeze();
+    (void)lib; (void)method; (void)map;
+    if (!R.inited) return fail("op_partition: op_init has not been called");
+    if (set && coords && coords->set == set && coords->dim == 3 && coords->impl && !D(coords)->is_int &&
+        set->impl && !S(set)->frozen && !D(coords)->pending.empty())
+        set_geometry(set, reinterpret_cast<const double*>(D(coords)->pending.data()));
+    return 0;
 }
 
 int op_renumber(op_map map)

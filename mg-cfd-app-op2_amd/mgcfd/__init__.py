@@ -341,8 +341,8 @@ class Runtime:
             raise MgcfdError("upload: size mismatch")
         _check(self.L.op_upload_data_char(dat, a.ctypes.data), "op_upload_data")
 
-    def partition(self):
-        _check(self.L.op_partition(b"PARMETIS", b"KWAY", None, None, None), "op_partition")
+    def partition(self, nodes=None, coords=None):
+        _check(self.L.op_partition(b"PARMETIS", b"KWAY", nodes, None, coords), "op_partition")
 
     def set_flux_variant(self, v: int):
         _check(self.L.mgcfd_set_flux_variant(v), "mgcfd_set_flux_variant")
@@ -495,6 +495,8 @@ class Solver:
             h.bnd_weights = rt.decl_dat(h.bnd_nodes, 3, m["bnd_weights"], "bnd_node_weights")
             h.coords = rt.decl_dat(h.nodes, 3, m["coords"], "node_coordinates")
             self.lv.append(h)
+        for h in self.lv:       # geometric node order (euler3d.cpp:340-377 passes the coordinates)
+            rt.partition(h.nodes, h.coords)
         for i in range(1, self.nlev):
             self.lv[i - 1].node_to_mg_node = rt.decl_map(self.lv[i - 1].nodes, self.lv[i].nodes, 1,
                                                          mesh[i - 1]["node_to_mg_node"], f"op_node-->mg_node_L{i}")
