@@ -452,41 +452,21 @@ __device__ __forceinline__ void part_apply(double* acc, unsigned code, const O i
     }
 }
 
-// LDS node records of the part kernels.  FP64: the raw state {rho, m, E}
-// (6 doubles), both endpoint states derived per edge -- the smaller record
-// lets more workgroups share a CU.  FP32: the derived state {rho, m, E, u,
-// p, |u| + c} (12 floats) is built once per staged node after the loads --
-// cheaper than deriving it per edge and still small.
+// LDS node records of the part kernels: the raw state {rho, m, E} (6 values,
+// FP64 or FP32); both endpoint states are derived per edge -- the small record
+// lets more workgroups share a CU, which pays more than deriving each state
+// once.
 template <typename R>
 struct PartRec;
 template <>
 struct PartRec<double> {
     static constexpr int kStride = 6;
-    static constexpr bool kDerived = false;
     static __device__ __forceinline__ PState<double> state(const double* r) { return part_state(r); }
-    static __device__ __forceinline__ void derive(double*) {}
 };
 template <>
 struct PartRec<float> {
-    static constexpr int kStride = 12;
-    static constexpr bool kDerived = true;
-    static __device__ __forceinline__ PState<float> state(const float* r)
-    {
-        const float4* q = reinterpret_cast<const float4*>(r);
-        const float4 a = q[0], b = q[1], c = q[2];
-        PState<float> t;
-        t.rho = a.x; t.mx = a.y; t.my = a.z; t.mz = a.w;
-        t.E = b.x; t.ux = b.y; t.uy = b.z; t.uz = b.w;
-        t.p = c.x; t.s = c.y;
-        return t;
-    }
-    static __device__ __forceinline__ void derive(float* r)
-    {
-        const PState<float> t = part_state32(r);
-        float4* w = reinterpret_cast<float4*>(r);
-        w[1] = make_float4(t.E, t.ux, t.uy, t.uz);
-        w[2] = make_float4(t.p, t.s, 0.f, 0.f);
-    }
+    static constexpr int kStride = 6;
+    static __device__ __forceinline__ PState<float> state(const float* r) { return part_state32(r); }
 };
 
 // LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
@@ -534,10 +514,6 @@ __global__ __launch_bounds__(T) void k_flux_part(
         rec[(P.cnt + h) * RS + v] = (R)var[(size_t)halo[P.h0 + h] * 5 + v];
     }
     __syncthreads();
-    if constexpr (PartRec<R>::kDerived) {
-        for (int s2 = t; s2 < P.cnt + P.nh; s2 += T) PartRec<R>::derive(rec + s2 * RS);
-        __syncthreads();
-    }
     // 2./3. edges
     const R sc = (R)k.sc;
     if constexpr (COLOURED) {

@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <atomic>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -111,6 +113,7 @@ struct InvPlan {
     unsigned pt_w32ver = 0;
     size_t pt_lds32 = 0;
     int pt_psize = kPart;            // max nodes per part (autotuned)
+    bool bk_built = false, pt_built = false;
     std::vector<int> h_ptr;          // host CSR kept for re-planning (dim-2 dense maps)
     std::vector<int2> h_inc2;
 };
@@ -425,23 +428,34 @@ static int freeze()
             if (m.impl && m.from == &s && m.to->impl && !S(m.to)->perm.empty() && (key == nullptr || m.dim > key->dim))
                 key = &m;
         if (!key) continue;
+        HostTimer tm("order", s.name);
         const MapImpl* km = M(key);
         const SetImpl* ts = S(key->to);
-        const int dim = key->dim;
-        std::vector<long long> k1(s.size);
+        const int dim = key->dim, nt = key->to->size;
+        // stable order by (lowest internal endpoint, highest): two counting
+        // passes, least significant key first
+        std::vector<int> lo(s.size), hi(s.size);
         for (int e = 0; e < s.size; e++) {
-            long long lo = ts->iperm[km->h_orig[(size_t)e * dim]];
-            long long hi = lo;
+            int a = ts->iperm[km->h_orig[(size_t)e * dim]], b = a;
             for (int c = 1; c < dim; c++) {
-                long long v = ts->iperm[km->h_orig[(size_t)e * dim + c]];
-                lo = std::min(lo, v);
-                hi = std::max(hi, v);
+                const int v = ts->iperm[km->h_orig[(size_t)e * dim + c]];
+                a = std::min(a, v);
+                b = std::max(b, v);
             }
-            k1[e] = lo * (long long)(key->to->size + 1) + hi;
+            lo[e] = a;
+            hi[e] = b;
         }
+        auto counting = [&](const std::vector<int>& key1, const std::vector<int>& in, std::vector<int>& out) {
+            std::vector<int> cnt(nt + 1, 0);
+            for (int e : in) cnt[key1[e] + 1]++;
+            for (int i = 0; i < nt; i++) cnt[i + 1] += cnt[i];
+            for (int e : in) out[cnt[key1[e]]++] = e;
+        };
+        std::vector<int> id(s.size), tmp(s.size);
+        std::iota(id.begin(), id.end(), 0);
+        counting(hi, id, tmp);
         si->perm.resize(s.size);
-        std::iota(si->perm.begin(), si->perm.end(), 0);
-        std::stable_sort(si->perm.begin(), si->perm.end(), [&](int x, int y) { return k1[x] < k1[y]; });
+        counting(lo, tmp, si->perm);
         si->iperm.assign(s.size, 0);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
     }
@@ -498,18 +512,46 @@ static int upload_dat(op_dat d)
 // prefix of the block's threads.
 constexpr int kHaloCap = kPart;
 
-static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
+// Runs f(i) for i in [0, n) on up to 16 host threads (the plan builders'
+// per-part work is independent); f must be thread-safe.
+template <typename F>
+static void parallel_for(int n, F&& f)
 {
-    const int nt = m->to->size;
-    std::vector<int2> blocks;
-    std::vector<int> hptr{0}, halo, jbase, pbase, jpref, eid;
+    const int nth = std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 64 || nth <= 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; t++)
+        pool.emplace_back([&] {
+            for (;;) {
+                const int i0 = next.fetch_add(8);
+                if (i0 >= n) break;
+                for (int i = i0; i < std::min(n, i0 + 8); i++) f(i);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+// Blocked-gather plan of one initial part (split while its halo exceeds
+// kHaloCap); offsets are local to the part and fixed up on concatenation.
+struct BlkOut {
+    std::vector<int2> blocks;           // {first node, count}
+    std::vector<int> nhalo, halo;       // per block halo count; halos concatenated
+    std::vector<int> npref, jpref;      // per block diagonal-offset count; offsets concatenated
+    std::vector<int> nent;              // per block entry count
     std::vector<unsigned> jslot;
+    std::vector<int> eid;
+    std::string err;
+};
+
+static void blocked_part(int lo0, int hi0, const std::vector<int>& ptr, const std::vector<int2>& inc2, BlkOut& out)
+{
+    std::vector<int2> work{make_int2(lo0, hi0)};
     std::vector<int> hb;
-    int max_halo = 0;
-    std::vector<int2> work;
-    const std::vector<int> pb = set_parts(m->to);
-    for (size_t k = pb.size() - 1; k-- > 0;) work.push_back(make_int2(pb[k], pb[k + 1]));
-    while (!work.empty()) {             // LIFO over reversed parts: blocks come out in node order
+    while (!work.empty()) {             // LIFO: blocks come out in node order
         const int lo = work.back().x, hi = work.back().y;
         work.pop_back();
         hb.clear();
@@ -525,19 +567,19 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
             work.push_back(make_int2(lo, mid));
             continue;
         }
-        blocks.push_back(make_int2(lo, hi - lo));
-        max_halo = std::max(max_halo, (int)hb.size());
         // degrees are non-increasing inside the block, so diagonal k covers
         // threads [0, cnt_k)
         const int maxdeg = ptr[lo + 1] - ptr[lo];
         for (int t = 1; t < hi - lo; t++)
-            if (ptr[lo + t + 1] - ptr[lo + t] > ptr[lo + t] - ptr[lo + t - 1])
-                return fail("internal error: node degrees not sorted inside a part");
-        jbase.push_back((int)jslot.size());
-        pbase.push_back((int)jpref.size());
+            if (ptr[lo + t + 1] - ptr[lo + t] > ptr[lo + t] - ptr[lo + t - 1]) {
+                out.err = "internal error: node degrees not sorted inside a part";
+                return;
+            }
+        out.blocks.push_back(make_int2(lo, hi - lo));
+        const size_t e0 = out.eid.size();
         int off = 0;
         for (int kd = 0; kd < maxdeg; kd++) {
-            jpref.push_back(off);
+            out.jpref.push_back(off);
             for (int t = 0; t < hi - lo; t++) {
                 const int nd = lo + t;
                 if (ptr[nd + 1] - ptr[nd] <= kd) break;
@@ -546,14 +588,48 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
                 const unsigned slot = (o >= lo && o < hi)
                     ? (unsigned)(o - lo)
                     : (unsigned)(kPart + (std::lower_bound(hb.begin(), hb.end(), o) - hb.begin()));
-                jslot.push_back(slot | ((unsigned)r.y & 0x80000000u));
-                eid.push_back(r.y & 0x7fffffff);
+                out.jslot.push_back(slot | ((unsigned)r.y & 0x80000000u));
+                out.eid.push_back(r.y & 0x7fffffff);
                 off++;
             }
         }
-        jpref.push_back(off);
-        halo.insert(halo.end(), hb.begin(), hb.end());
-        hptr.push_back((int)halo.size());
+        out.jpref.push_back(off);
+        out.npref.push_back(maxdeg + 1);
+        out.nent.push_back((int)(out.eid.size() - e0));
+        out.nhalo.push_back((int)hb.size());
+        out.halo.insert(out.halo.end(), hb.begin(), hb.end());
+    }
+}
+
+static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
+{
+    HostTimer tm("blocked plan", m->name);
+    const std::vector<int> pb = set_parts(m->to);
+    const int np = (int)pb.size() - 1;
+    std::vector<BlkOut> outs(std::max(np, 0));
+    parallel_for(np, [&](int k) { blocked_part(pb[k], pb[k + 1], ptr, inc2, outs[k]); });
+    std::vector<int2> blocks;
+    std::vector<int> hptr{0}, halo, jbase, pbase, jpref, eid;
+    std::vector<unsigned> jslot;
+    int max_halo = 0;
+    for (auto& o : outs) {
+        if (!o.err.empty()) return fail("%s", o.err.c_str());
+        size_t hp = 0, pp = 0, ep = 0;
+        for (size_t b = 0; b < o.blocks.size(); b++) {
+            blocks.push_back(o.blocks[b]);
+            jbase.push_back((int)(jslot.size() + ep));
+            pbase.push_back((int)(jpref.size() + pp));
+            hp += o.nhalo[b];
+            hptr.push_back((int)(halo.size() + hp));
+            pp += o.npref[b];
+            ep += o.nent[b];
+            max_halo = std::max(max_halo, o.nhalo[b]);
+        }
+        halo.insert(halo.end(), o.halo.begin(), o.halo.end());
+        jpref.insert(jpref.end(), o.jpref.begin(), o.jpref.end());
+        jslot.insert(jslot.end(), o.jslot.begin(), o.jslot.end());
+        eid.insert(eid.end(), o.eid.begin(), o.eid.end());
+        o = BlkOut();
     }
     // invariants the kernel relies on
     for (size_t bi = 0; bi < blocks.size(); bi++) {
@@ -594,19 +670,23 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
 // share an owned node, and sorted by colour.
 constexpr int kPartSlots = 704;   // LDS node records per part (10-bit slots)
 
-static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
-{
-    const int nt = m->to->size;
+// Part plan of one initial part (split while owned + halo nodes exceed
+// kPartSlots or nodes exceed the part size); h0/e0 local to the part.
+struct PartOut {
     std::vector<PartInfo> parts;
-    std::vector<int> halo, hb, eid;
+    std::vector<int> halo, eid;
     std::vector<unsigned> code;
-    std::vector<int2> work;
+    std::string err;
+};
+
+static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, const std::vector<int2>& inc2,
+                      PartOut& out)
+{
+    std::vector<int2> work{make_int2(lo0, hi0)};
+    std::vector<int> hb;
     std::vector<unsigned long long> used(kPart);   // per owned node: colours taken
     struct LE { int a, b, e; unsigned flags; int col; };
     std::vector<LE> le;
-    int rec_cap = 0, max_ne = 0, max_col = 0;
-    const std::vector<int> pb = set_parts(m->to);
-    for (size_t k = pb.size() - 1; k-- > 0;) work.push_back(make_int2(pb[k], pb[k + 1]));
     while (!work.empty()) {
         const int lo = work.back().x, hi = work.back().y;
         work.pop_back();
@@ -617,8 +697,12 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
         }
         std::sort(hb.begin(), hb.end());
         hb.erase(std::unique(hb.begin(), hb.end()), hb.end());
-        if ((hi - lo) + (int)hb.size() > kPartSlots || (hi - lo) > P.pt_psize) {
-            if (hi - lo <= 8) return fail("edge-flux part plan: node %d has more than %d neighbours", lo, kPartSlots);
+        if ((hi - lo) + (int)hb.size() > kPartSlots || (hi - lo) > psize) {
+            if (hi - lo <= 8) {
+                out.err = "edge-flux part plan: node " + std::to_string(lo) + " has more than " +
+                          std::to_string(kPartSlots) + " neighbours";
+                return;
+            }
             const int mid = lo + (hi - lo) / 2;
             work.push_back(make_int2(mid, hi));
             work.push_back(make_int2(lo, mid));
@@ -650,7 +734,10 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
             unsigned long long u = 0;
             if (x.flags & 1u) u |= used[x.a - lo];
             if (x.flags & 2u) u |= used[x.b - lo];
-            if (~u == 0ull) return fail("edge-flux part plan: more than 64 edge colours in a part");
+            if (~u == 0ull) {
+                out.err = "edge-flux part plan: more than 64 edge colours in a part";
+                return;
+            }
             const int c = __builtin_ctzll(~u);
             x.col = c;
             if (x.flags & 1u) used[x.a - lo] |= 1ull << c;
@@ -661,22 +748,49 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
         PartInfo pi;
         pi.n0 = lo;
         pi.cnt = hi - lo;
-        pi.h0 = (int)halo.size();
+        pi.h0 = (int)out.halo.size();
         pi.nh = (int)hb.size();
-        pi.e0 = (int)code.size();
+        pi.e0 = (int)out.code.size();
         pi.ne = (int)le.size();
         pi.ncol = ncol;
         pi.pad = 0;
         for (const auto& x : le) {
             const unsigned sa = (unsigned)slot(x.a), sb = (unsigned)slot(x.b);
-            code.push_back(sa | (sb << 10) | (x.flags << 20) | ((unsigned)x.col << 22));
-            eid.push_back(x.e);
+            out.code.push_back(sa | (sb << 10) | (x.flags << 20) | ((unsigned)x.col << 22));
+            out.eid.push_back(x.e);
         }
-        halo.insert(halo.end(), hb.begin(), hb.end());
-        parts.push_back(pi);
-        rec_cap = std::max(rec_cap, pi.cnt + pi.nh);
-        max_ne = std::max(max_ne, pi.ne);
-        max_col = std::max(max_col, ncol);
+        out.halo.insert(out.halo.end(), hb.begin(), hb.end());
+        out.parts.push_back(pi);
+    }
+}
+
+static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
+{
+    HostTimer tm("part plan", m->name);
+    const int nt = m->to->size;
+    const std::vector<int> pb = set_parts(m->to);
+    const int np = (int)pb.size() - 1;
+    std::vector<PartOut> outs(std::max(np, 0));
+    const int psize = P.pt_psize;
+    parallel_for(np, [&](int k) { part_part(pb[k], pb[k + 1], psize, ptr, inc2, outs[k]); });
+    std::vector<PartInfo> parts;
+    std::vector<int> halo, eid;
+    std::vector<unsigned> code;
+    int rec_cap = 0, max_ne = 0, max_col = 0;
+    for (auto& o : outs) {
+        if (!o.err.empty()) return fail("%s", o.err.c_str());
+        for (PartInfo pi : o.parts) {
+            pi.h0 += (int)halo.size();
+            pi.e0 += (int)code.size();
+            parts.push_back(pi);
+            rec_cap = std::max(rec_cap, pi.cnt + pi.nh);
+            max_ne = std::max(max_ne, pi.ne);
+            max_col = std::max(max_col, pi.ncol);
+        }
+        halo.insert(halo.end(), o.halo.begin(), o.halo.end());
+        code.insert(code.end(), o.code.begin(), o.code.end());
+        eid.insert(eid.end(), o.eid.begin(), o.eid.end());
+        o = PartOut();
     }
     // invariants the kernel relies on
     long long covered = 0;
@@ -713,7 +827,7 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     P.pt_max_col = max_col;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
     P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * PartRec<float>::kStride * sizeof(float) + 7) / 8 +
-                                   (size_t)kPart * 5);
+                                   (size_t)kPart * 5);   // FP32 records, FP64 accumulators
     return 0;
 }
 
@@ -770,9 +884,7 @@ static int build_inverse(op_map m)
     if (dim == 2) {
         HIPCHK(hipMalloc(&P.d_inc2, sizeof(int2) * std::max<size_t>(1, inc2.size())));
         if (!inc2.empty()) HIPCHK(hipMemcpy(P.d_inc2, inc2.data(), sizeof(int2) * inc2.size(), hipMemcpyHostToDevice));
-        if (P.dense) {
-            TRY(build_blocked(P, m, ptr, inc2));
-            TRY(build_partplan(P, m, ptr, inc2));
+        if (P.dense) {          // flux plans are built on first use (ensure_flux_plan)
             P.h_ptr = std::move(ptr);
             P.h_inc2 = std::move(inc2);
         }
@@ -804,11 +916,14 @@ static int set_part_nodes(op_map m, int nodes)
     TRY(build_inverse(m));
     InvPlan& P = M(m)->inv;
     if (!P.dense || P.h_ptr.empty()) return fail("set_part_nodes: map %s has no part plan", m->name);
-    if (P.pt_psize == nodes && P.pt_parts > 0) return 0;
+    if (P.pt_psize == nodes && P.pt_built) return 0;
     HIPCHK(hipStreamSynchronize(R.stream));
     free_partplan(P);
+    P.pt_built = false;
     P.pt_psize = nodes;
-    return build_partplan(P, m, P.h_ptr, P.h_inc2);
+    TRY(build_partplan(P, m, P.h_ptr, P.h_inc2));
+    P.pt_built = true;
+    return 0;
 }
 
 static void free_invplan(InvPlan& P)
@@ -849,6 +964,27 @@ static void release_set(op_set s)
     delete S(s);
     s->impl = nullptr;
     s->size = 0;
+}
+
+// Builds, on first use, the plan the given edge-flux variant launches on:
+// the blocked-gather plan (GATHER_*) or the part plan (PART_*, MIXED_*).
+static int ensure_flux_plan(op_map m, int variant)
+{
+    TRY(build_inverse(m));
+    InvPlan& P = M(m)->inv;
+    if (!P.dense || P.h_ptr.empty()) return 0;
+    const bool gather = variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST;
+    const bool part = variant == MGCFD_FLUX_PART_COLOURED || variant == MGCFD_FLUX_PART_ATOMIC ||
+                      variant == MGCFD_FLUX_MIXED_COLOURED || variant == MGCFD_FLUX_MIXED_ATOMIC;
+    if (gather && !P.bk_built) {
+        TRY(build_blocked(P, m, P.h_ptr, P.h_inc2));
+        P.bk_built = true;
+    }
+    if (part && !P.pt_built) {
+        TRY(build_partplan(P, m, P.h_ptr, P.h_inc2));
+        P.pt_built = true;
+    }
+    return 0;
 }
 
 static int packed_weights(op_dat w, const double4** out)
@@ -1006,7 +1142,7 @@ static int prepare_flux(int variant, op_map map, op_dat w)
     const double4* wp = nullptr;
     TRY(packed_weights(w, &wp));
     if (variant == MGCFD_FLUX_ATOMIC) return 0;
-    TRY(build_inverse(map));
+    TRY(ensure_flux_plan(map, variant));
     InvPlan& P = M(map)->inv;
     TRY(refresh_plan_weights(variant, P, w));
     return 0;
@@ -1023,7 +1159,7 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
                                                          dptr(flux), R.k, status);
         return check_launch();
     }
-    TRY(build_inverse(map));
+    TRY(ensure_flux_plan(map, variant));
     InvPlan& P = M(map)->inv;
     if (P.n_targets == 0) return 0;
     TRY(refresh_plan_weights(variant, P, w));
@@ -1338,7 +1474,7 @@ int op_rank(int* rank)
 int mgcfd_plan_stats(op_map map, int* stats)
 {
     if (!map || !stats) return fail("mgcfd_plan_stats: null argument");
-    TRY(build_inverse(map));
+    TRY(ensure_flux_plan(map, MGCFD_FLUX_GATHER_EXACT));
     const InvPlan& P = M(map)->inv;
     stats[0] = P.bk_blocks;
     stats[1] = kPart + P.bk_max_halo;
