@@ -470,10 +470,11 @@ struct PartRec<float> {
 };
 
 // LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
-// accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
+// accumulators [cnt * 5].  MAXJ * T >= 5 * rec_cap (staged entries per
+// thread); kPart <= T (owned entries: 5 per thread).  R = double: the FP64 kernel; R = float: mixed
 // precision (records, weights and increment arithmetic in FP32, each
 // increment converted to FP64 and accumulated in FP64 from the FP64 flux).
-template <typename R, int T, int EPT, bool COLOURED>
+template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
 __global__ __launch_bounds__(T) void k_flux_part(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
@@ -502,16 +503,41 @@ __global__ __launch_bounds__(T) void k_flux_part(
             w2[i] = ew[2 * nloc + e];
         }
     }
-    const double* vo = var + (size_t)P.n0 * 5;
+    // staged state: flat entry j = slot * 5 + v over the owned slots [0, cnt)
+    // and the halo slots [cnt, cnt + nh); every index load, then every state
+    // and flux load, is issued before the first LDS store
     const double* fo = flux + (size_t)P.n0 * 5;
-    for (int j = t; j < P.cnt * 5; j += T) {
-        const int nd = j / 5, v = j - nd * 5;
-        rec[nd * RS + v] = (R)vo[j];
-        acc[j] = fo[j];
+    const int nst = (P.cnt + P.nh) * 5;
+    int src[MAXJ];
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) {
+        const int j = t + i * T;
+        src[i] = -1;
+        if (j < nst) {
+            const int sl = j / 5, v = j - sl * 5;
+            src[i] = (sl < P.cnt ? P.n0 + sl : halo[P.h0 + sl - P.cnt]) * 5 + v;
+        }
     }
-    for (int j = t; j < P.nh * 5; j += T) {
-        const int h = j / 5, v = j - h * 5;
-        rec[(P.cnt + h) * RS + v] = (R)var[(size_t)halo[P.h0 + h] * 5 + v];
+    double st[MAXJ], fl[5];
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) st[i] = src[i] >= 0 ? var[src[i]] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int j = t + i * T;
+        fl[i] = j < P.cnt * 5 ? fo[j] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) {
+        const int j = t + i * T;
+        if (j < nst) {
+            const int sl = j / 5, v = j - sl * 5;
+            rec[sl * RS + v] = (R)st[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int j = t + i * T;
+        if (j < P.cnt * 5) acc[j] = fl[i];
     }
     __syncthreads();
     // 2./3. edges

@@ -1109,17 +1109,26 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
     return check_launch();
 }
 
-template <typename Real, int EPT>
+template <typename Real, int EPT, int MAXJ>
 static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                          const int* status)
 {
     if (coloured)
-        k_flux_part<Real, kBlock, EPT, true><<<P.pt_parts, kBlock, lds, R.stream>>>(
+        k_flux_part<Real, kBlock, EPT, MAXJ, true><<<P.pt_parts, kBlock, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     else
-        k_flux_part<Real, kBlock, EPT, false><<<P.pt_parts, kBlock, lds, R.stream>>>(
+        k_flux_part<Real, kBlock, EPT, MAXJ, false><<<P.pt_parts, kBlock, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     return check_launch();
+}
+
+template <typename Real, int EPT>
+static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                         const int* status)
+{
+    static_assert(kPartSlots * 5 <= 16 * kBlock, "staged entries exceed the kernel's registers");
+    if (P.pt_rec_cap * 5 <= 12 * kBlock) return launch_part_t<Real, EPT, 12>(coloured, P, w, lds, var, flux, status);
+    return launch_part_t<Real, EPT, 16>(coloured, P, w, lds, var, flux, status);
 }
 
 // Real = double: the FP64 part kernel; Real = float: mixed precision.
@@ -1128,10 +1137,10 @@ static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t ld
                        const int* status)
 {
     const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
-    if (ept <= 4) return launch_part_t<Real, 4>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 6) return launch_part_t<Real, 6>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 8) return launch_part_t<Real, 8>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 12) return launch_part_t<Real, 12>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 4) return launch_part_e<Real, 4>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 6) return launch_part_e<Real, 6>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 8) return launch_part_e<Real, 8>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 12) return launch_part_e<Real, 12>(coloured, P, w, lds, var, flux, status);
     return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
 }
 
