@@ -471,7 +471,7 @@ struct PartRec<float> {
 
 // LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
 // accumulators [cnt * 5].  MAXJ * T >= 5 * rec_cap (staged entries per
-// thread); kPart <= T (owned entries: 5 per thread).  R = double: the FP64 kernel; R = float: mixed
+// thread); the halo list holds one entry past the last part's halo.  R = double: the FP64 kernel; R = float: mixed
 // precision (records, weights and increment arithmetic in FP32, each
 // increment converted to FP64 and accumulated in FP64 from the FP64 flux).
 template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
@@ -487,7 +487,23 @@ __global__ __launch_bounds__(T) void k_flux_part(
     const int t = threadIdx.x;
     R* rec = reinterpret_cast<R*>(sm);
     double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
-    // 1. loads
+    // 1. loads.  Staged state: flat entry j = slot * 5 + v over the owned
+    // slots [0, cnt) and the halo slots [cnt, cnt + nh).  The halo index
+    // loads are unconditional (clamped into the part's halo list, which the
+    // plan pads by one entry) so that they issue back to back under one wait
+    // instead of one wait per divergent branch; the edge codes and weights
+    // follow, then every state and flux load, all before the first LDS store.
+    const double* fo = flux + (size_t)P.n0 * 5;
+    const int nst = (P.cnt + P.nh) * 5;
+    const int hmax = max(P.nh - 1, 0);
+    int src[MAXJ];
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) {
+        const int j = t + i * T;
+        const int sl = j / 5, v = j - sl * 5;
+        const int hn = halo[P.h0 + min(max(sl - P.cnt, 0), hmax)];
+        src[i] = (sl < P.cnt ? P.n0 + sl : hn) * 5 + v;   // a valid state index for every j
+    }
     unsigned code[EPT];
     R w0[EPT], w1[EPT], w2[EPT];
 #pragma unroll
@@ -503,29 +519,12 @@ __global__ __launch_bounds__(T) void k_flux_part(
             w2[i] = ew[2 * nloc + e];
         }
     }
-    // staged state: flat entry j = slot * 5 + v over the owned slots [0, cnt)
-    // and the halo slots [cnt, cnt + nh); every index load, then every state
-    // and flux load, is issued before the first LDS store
-    const double* fo = flux + (size_t)P.n0 * 5;
-    const int nst = (P.cnt + P.nh) * 5;
-    int src[MAXJ];
+    constexpr int FL = (5 * kPart + T - 1) / T;   // owned flux entries per thread
+    double st[MAXJ], fl[FL];
 #pragma unroll
-    for (int i = 0; i < MAXJ; i++) {
-        const int j = t + i * T;
-        src[i] = -1;
-        if (j < nst) {
-            const int sl = j / 5, v = j - sl * 5;
-            src[i] = (sl < P.cnt ? P.n0 + sl : halo[P.h0 + sl - P.cnt]) * 5 + v;
-        }
-    }
-    double st[MAXJ], fl[5];
+    for (int i = 0; i < MAXJ; i++) st[i] = var[src[i]];
 #pragma unroll
-    for (int i = 0; i < MAXJ; i++) st[i] = src[i] >= 0 ? var[src[i]] : 0.0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        const int j = t + i * T;
-        fl[i] = j < P.cnt * 5 ? fo[j] : 0.0;
-    }
+    for (int i = 0; i < FL; i++) fl[i] = fo[min(t + i * T, P.cnt * 5 - 1)];
 #pragma unroll
     for (int i = 0; i < MAXJ; i++) {
         const int j = t + i * T;
@@ -535,7 +534,7 @@ __global__ __launch_bounds__(T) void k_flux_part(
         }
     }
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
+    for (int i = 0; i < FL; i++) {
         const int j = t + i * T;
         if (j < P.cnt * 5) acc[j] = fl[i];
     }

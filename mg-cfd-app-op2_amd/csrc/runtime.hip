@@ -816,6 +816,7 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
         return 0;
     };
     TRY(up(P.d_pt_info, parts));
+    halo.push_back(0);   // pad: the kernel's clamped halo index loads stay in bounds when nh == 0
     TRY(up(P.d_pt_halo, halo));
     TRY(up(P.d_pt_code, code));
     TRY(up(P.d_pt_eid, eid));
@@ -1109,39 +1110,46 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
     return check_launch();
 }
 
-template <typename Real, int EPT, int MAXJ>
+template <typename Real, int T, int EPT, int MAXJ>
 static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                          const int* status)
 {
     if (coloured)
-        k_flux_part<Real, kBlock, EPT, MAXJ, true><<<P.pt_parts, kBlock, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, MAXJ, true><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     else
-        k_flux_part<Real, kBlock, EPT, MAXJ, false><<<P.pt_parts, kBlock, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, MAXJ, false><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
     return check_launch();
 }
 
-template <typename Real, int EPT>
+// Staged entries per thread: the smaller register array when the part plan's
+// largest part allows it.
+template <typename Real, int T, int EPT>
 static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                          const int* status)
 {
-    static_assert(kPartSlots * 5 <= 16 * kBlock, "staged entries exceed the kernel's registers");
-    if (P.pt_rec_cap * 5 <= 12 * kBlock) return launch_part_t<Real, EPT, 12>(coloured, P, w, lds, var, flux, status);
-    return launch_part_t<Real, EPT, 16>(coloured, P, w, lds, var, flux, status);
+    constexpr int kSmall = 3072 / T, kLarge = 4096 / T;
+    static_assert(kPartSlots * 5 <= kLarge * T, "staged entries exceed the kernel's registers");
+    if (P.pt_rec_cap * 5 <= kSmall * T)
+        return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status);
+    return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status);
 }
 
-// Real = double: the FP64 part kernel; Real = float: mixed precision.
+// Real = double: the FP64 part kernel, 256 threads per part; Real = float:
+// mixed precision, whose smaller register footprint lets 512 threads (8
+// waves) share one part, which shortens each workgroup's edge phase.
 template <typename Real>
 static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                        const int* status)
 {
-    const int ept = (P.pt_max_ne + kBlock - 1) / kBlock;
-    if (ept <= 4) return launch_part_e<Real, 4>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 6) return launch_part_e<Real, 6>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 8) return launch_part_e<Real, 8>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 12) return launch_part_e<Real, 12>(coloured, P, w, lds, var, flux, status);
-    return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 12 * kBlock);
+    constexpr int T = sizeof(Real) == 4 ? 512 : 256;
+    const int ept = (P.pt_max_ne + T - 1) / T;
+    if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status);
+    return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 3072);
 }
 
 // Plans and packed weights a flux launch needs; allocates and uploads, so it
