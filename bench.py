@@ -55,10 +55,10 @@ VARIANT_KERNELS = {
     "gather_fast": r"k_flux_blocked_pf<false",
     "atomic": r"k_flux_atomic\(",
     "nodal_exact": r"k_flux_gather<true>",
-    "part_coloured": r"k_flux_part<double, \d+, \d+, true",
-    "part_atomic": r"k_flux_part<double, \d+, \d+, false",
-    "mixed_coloured": r"k_flux_part<float, \d+, \d+, true",
-    "mixed_atomic": r"k_flux_part<float, \d+, \d+, false",
+    "part_coloured": r"k_flux_part<double, (?:\d+, )+true",
+    "part_atomic": r"k_flux_part<double, (?:\d+, )+false",
+    "mixed_coloured": r"k_flux_part<float, (?:\d+, )+true",
+    "mixed_atomic": r"k_flux_part<float, (?:\d+, )+false",
 }
 
 

@@ -469,43 +469,56 @@ struct PartRec<float> {
     static __device__ __forceinline__ PState<float> state(const float* r) { return part_state32(r); }
 };
 
-// LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
-// accumulators [cnt * 5].  MAXJ * T >= 5 * rec_cap (staged entries per
-// thread); the halo list holds one entry past the last part's halo.  R = double: the FP64 kernel; R = float: mixed
-// precision (records, weights and increment arithmetic in FP32, each
-// increment converted to FP64 and accumulated in FP64 from the FP64 flux).
-template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
-__global__ __launch_bounds__(T) void k_flux_part(
-    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
-    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
-    double* __restrict__ flux, Consts k, const int* __restrict__ status)
+// Staging of one part, shared by the part kernels.  Staged state: flat
+// entry j = slot * 5 + v over the owned slots [0, cnt) and the halo slots
+// [cnt, cnt + nh), MAXJ entries per thread (MAXJ * T >= 5 * rec_cap).  The
+// halo index loads are unconditional (clamped into the part's halo list,
+// which the plan pads by one entry) so that they issue back to back under one
+// wait instead of one wait per divergent branch (part_halo_entries, then
+// part_stage_select).
+//
+// Marks a loaded register as needed here: the compiler waits for its load at
+// this point and cannot sink the load into a later divergent branch (where it
+// would be issued alone under its own wait).
+template <typename V>
+__device__ __forceinline__ void part_ready(const V& v)
 {
-    constexpr int RS = PartRec<R>::kStride;
-    extern __shared__ double sm[];
-    if (halted(status)) return;
-    const PartInfo P = parts[xcd_block()];
-    const int t = threadIdx.x;
-    R* rec = reinterpret_cast<R*>(sm);
-    double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
-    // 1. loads.  Staged state: flat entry j = slot * 5 + v over the owned
-    // slots [0, cnt) and the halo slots [cnt, cnt + nh).  The halo index
-    // loads are unconditional (clamped into the part's halo list, which the
-    // plan pads by one entry) so that they issue back to back under one wait
-    // instead of one wait per divergent branch; the edge codes and weights
-    // follow, then every state and flux load, all before the first LDS store.
-    const double* fo = flux + (size_t)P.n0 * 5;
-    const int nst = (P.cnt + P.nh) * 5;
+    asm volatile("" ::"v"(v));
+}
+
+// Raw halo-list entries of the staged slots (index loads only; the
+// select happens in part_stage_select once they are needed).
+template <int T, int MAXJ>
+__device__ __forceinline__ void part_halo_entries(const PartInfo& P, const int* __restrict__ halo, int t,
+                                                  int hn[MAXJ])
+{
     const int hmax = max(P.nh - 1, 0);
-    int src[MAXJ];
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) {
+        const int sl = (t + i * T) / 5;
+        hn[i] = halo[P.h0 + min(max(sl - P.cnt, 0), hmax)];
+    }
+}
+
+template <int T, int MAXJ>
+__device__ __forceinline__ void part_stage_select(const PartInfo& P, int t, const int hn[MAXJ], int src[MAXJ])
+{
+#pragma unroll
+    for (int i = 0; i < MAXJ; i++) part_ready(hn[i]);
 #pragma unroll
     for (int i = 0; i < MAXJ; i++) {
         const int j = t + i * T;
         const int sl = j / 5, v = j - sl * 5;
-        const int hn = halo[P.h0 + min(max(sl - P.cnt, 0), hmax)];
-        src[i] = (sl < P.cnt ? P.n0 + sl : hn) * 5 + v;   // a valid state index for every j
+        src[i] = (sl < P.cnt ? P.n0 + sl : hn[i]) * 5 + v;
     }
-    unsigned code[EPT];
-    R w0[EPT], w1[EPT], w2[EPT];
+}
+
+// Local-edge codes and weights of one part, EPT per thread (code 0: none).
+template <typename R, int T, int EPT>
+__device__ __forceinline__ void part_load_edges(const PartInfo& P, const unsigned* __restrict__ ecode,
+                                                const R* __restrict__ ew, long long nloc, int t, unsigned code[EPT],
+                                                R w0[EPT], R w1[EPT], R w2[EPT])
+{
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         const int j = t + i * T;
@@ -519,12 +532,26 @@ __global__ __launch_bounds__(T) void k_flux_part(
             w2[i] = ew[2 * nloc + e];
         }
     }
-    constexpr int FL = (5 * kPart + T - 1) / T;   // owned flux entries per thread
-    double st[MAXJ], fl[FL];
+}
+
+// State entries and the owned nodes' old flux (FL entries per thread).
+template <int T, int MAXJ, int FL>
+__device__ __forceinline__ void part_load_state(const PartInfo& P, const double* __restrict__ var,
+                                                const double* __restrict__ flux, int t, const int src[MAXJ],
+                                                double st[MAXJ], double fl[FL])
+{
+    const double* fo = flux + (size_t)P.n0 * 5;
 #pragma unroll
     for (int i = 0; i < MAXJ; i++) st[i] = var[src[i]];
 #pragma unroll
     for (int i = 0; i < FL; i++) fl[i] = fo[min(t + i * T, P.cnt * 5 - 1)];
+}
+
+template <typename R, int T, int MAXJ, int FL, int RS>
+__device__ __forceinline__ void part_store_lds(const PartInfo& P, int t, const double st[MAXJ],
+                                               const double fl[FL], R* rec, double* acc)
+{
+    const int nst = (P.cnt + P.nh) * 5;
 #pragma unroll
     for (int i = 0; i < MAXJ; i++) {
         const int j = t + i * T;
@@ -538,6 +565,55 @@ __global__ __launch_bounds__(T) void k_flux_part(
         const int j = t + i * T;
         if (j < P.cnt * 5) acc[j] = fl[i];
     }
+}
+
+// The part's local edges, each increment added to its owned endpoints with
+// LDS atomics.
+template <typename R, int EPT, int RS>
+__device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, const unsigned code[EPT],
+                                                  const R w0[EPT], const R w1[EPT], const R w2[EPT], R sc)
+{
+#pragma unroll
+    for (int i = 0; i < EPT; i++) {
+        if (!code[i]) continue;
+        const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
+        const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
+        R inc[5];
+        part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc);
+        part_apply(acc, code[i], inc);
+    }
+}
+
+// LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
+// accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
+// precision (records, weights and increment arithmetic in FP32, each
+// increment converted to FP64 and accumulated in FP64 from the FP64 flux).
+// One workgroup per part.
+template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
+__global__ __launch_bounds__(T) void k_flux_part(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
+    double* __restrict__ flux, Consts k, const int* __restrict__ status)
+{
+    constexpr int RS = PartRec<R>::kStride;
+    constexpr int FL = (5 * kPart + T - 1) / T;
+    extern __shared__ double sm[];
+    if (halted(status)) return;
+    const PartInfo P = parts[xcd_block()];
+    const int t = threadIdx.x;
+    R* rec = reinterpret_cast<R*>(sm);
+    double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
+    // 1. loads: halo indices, edge codes and weights, then state and flux
+    int src[MAXJ];
+    unsigned code[EPT];
+    R w0[EPT], w1[EPT], w2[EPT];
+    double st[MAXJ], fl[FL];
+    int hn[MAXJ];
+    part_halo_entries<T, MAXJ>(P, halo, t, hn);
+    part_load_edges<R, T, EPT>(P, ecode, ew, nloc, t, code, w0, w1, w2);
+    part_stage_select<T, MAXJ>(P, t, hn, src);
+    part_load_state<T, MAXJ, FL>(P, var, flux, t, src, st, fl);
+    part_store_lds<R, T, MAXJ, FL, RS>(P, t, st, fl, rec, acc);
     __syncthreads();
     // 2./3. edges
     const R sc = (R)k.sc;
@@ -557,15 +633,7 @@ __global__ __launch_bounds__(T) void k_flux_part(
             __syncthreads();
         }
     } else {
-#pragma unroll
-        for (int i = 0; i < EPT; i++) {
-            if (!code[i]) continue;
-            const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
-            const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
-            R inc[5];
-            part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc);
-            part_apply(acc, code[i], inc);
-        }
+        part_edges_atomic<R, EPT, RS>(rec, acc, code, w0, w1, w2, sc);
         __syncthreads();
     }
     // 4. store
