@@ -793,15 +793,12 @@ __global__ void k_pack_weights(int n, const double* __restrict__ w, double4* __r
 __global__ __launch_bounds__(kBlock) void k_bnd_flux(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int* __restrict__ ent, const int* __restrict__ group, const double* __restrict__ bw,
-    const double* __restrict__ var, double* __restrict__ flux, Consts k0, const int* __restrict__ status,
-    const Consts* __restrict__ kseg, const int* __restrict__ tseg)
+    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n_targets) return;
-    // batched solves: far-field constants of the target's own mesh
-    const Consts& k = kseg ? kseg[tseg[tid]] : k0;
     const int n = targets[tid];
     double U[5], acc[5];
     load5(var, n, U);
@@ -976,9 +973,10 @@ __global__ __launch_bounds__(kBlock) void k_min_part(int n, const double* __rest
 // set, a negative minimum raises abort code 1 (euler3d.cpp:480-484).
 __global__ __launch_bounds__(kBlock) void k_min_final(int np, const double* __restrict__ part,
                                                        double init, double* __restrict__ out,
-                                                       int* __restrict__ status)
+                                                       int* __restrict__ status, int* __restrict__ zero = nullptr)
 {
     if (status && *status != 0) return;
+    if (zero && threadIdx.x == 0) *zero = 0;   // a counter the step's later kernels accumulate into
     double m = DBL_MAX;
     for (int i = threadIdx.x; i < np; i += kBlock) m = (part[i] < m) ? part[i] : m;
     m = block_min(m);
@@ -1018,6 +1016,121 @@ __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double
     store5(var, i, u);
     const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     store5(flux, i, z);
+}
+
+// One Runge-Kutta stage of a multigrid smoothing step after its edge flux,
+// fused into one pass over the nodes: the node's boundary and far-field
+// increments (flux.h:14-39, k_bnd_flux's arithmetic in boundary-element
+// order), on the first stage the step factor (time_stepping_kernels.h:43-64),
+// the stage update (time_stepping_kernels.h:66-86) and on the last stage the
+// residual with its per-workgroup sum of squares and NaN/Inf count
+// (validation.h:27-44, 102-115).  Each node sees the same operations in the
+// same order as with the separate loops.
+struct StageArgs {
+    int n, rk;
+    const int4* blk;           // batched solves: {segment, first, end} per workgroup; null: plain blocks over n
+    const double* dmin;        // minimum dt: [0], or per segment
+    const double* vol;
+    double* sf;
+    double* flux;
+    const double* old;
+    double* var;
+    const int* bptr;           // boundary elements of node i: bent[bptr[i] .. bptr[i + 1]) (null: none)
+    const int* bent;
+    const int* bgroup;
+    const double* bw;
+    const Consts* kseg;        // batched solves: far-field constants per segment
+    double* res;               // last stage: residual, or null
+    double* part;              // last stage, finest level: per-workgroup sum of squares
+    int* bad;                  // ... NaN/Inf count: one counter (atomic) ...
+    int* badpart;              // ... or per workgroup
+    const int* status;
+};
+
+__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
+{
+#pragma clang fp contract(off)
+    if (halted(a.status)) return;
+    int i, lim, seg = 0;
+    if (a.blk) {
+        const int4 B = a.blk[blockIdx.x];
+        seg = B.x;
+        i = B.y + threadIdx.x;
+        lim = B.z;
+    } else {
+        i = blockIdx.x * blockDim.x + threadIdx.x;
+        lim = a.n;
+    }
+    double s = 0.0;
+    int nb = 0;
+    if (i < lim) {
+        double f[5], o[5], u[5];
+        load5(a.flux, i, f);
+        if (a.bptr) {
+            const int q0 = a.bptr[i], q1 = a.bptr[i + 1];
+            if (q0 < q1) {
+                const Consts& k = a.kseg ? a.kseg[seg] : k0;
+                double U[5];
+                load5(a.var, i, U);
+                Th b;
+                thermo_exact(U, b);
+                for (int q = q0; q < q1; q++) {
+                    const int e = a.bent[q];
+                    const int g = a.bgroup[e];
+                    const double w0 = a.bw[3 * (size_t)e], w1 = a.bw[3 * (size_t)e + 1], w2 = a.bw[3 * (size_t)e + 2];
+                    if (g <= 2) {
+                        f[0] = f[0] + 0.0;
+                        f[1] = f[1] + w0 * b.p;
+                        f[2] = f[2] + w1 * b.p;
+                        f[3] = f[3] + w2 * b.p;
+                        f[4] = f[4] + 0.0;
+                    } else if (g <= 7) {
+                        const double fx = 0.5 * w0, fy = 0.5 * w1, fz = 0.5 * w2;
+                        f[0] += fx * (k.ffv[1] + b.mx) + fy * (k.ffv[2] + b.my) + fz * (k.ffv[3] + b.mz);
+                        f[4] += fx * (k.ffde[0] + b.fdx) + fy * (k.ffde[1] + b.fdy) + fz * (k.ffde[2] + b.fdz);
+                        f[1] += fx * (k.ffmx[0] + b.fxx) + fy * (k.ffmx[1] + b.fxy) + fz * (k.ffmx[2] + b.fxz);
+                        f[2] += fx * (k.ffmy[0] + b.fxy) + fy * (k.ffmy[1] + b.fyy) + fz * (k.ffmy[2] + b.fyz);
+                        f[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
+                    }
+                }
+            }
+        }
+        double sfi;
+        if (a.rk == 0) {
+            sfi = a.dmin[seg] / a.vol[i];
+            a.sf[i] = sfi;
+        } else {
+            sfi = a.sf[i];
+        }
+        const double factor = sfi / (double)(3 + 1 - a.rk);
+        load5(a.old, i, o);
+#pragma unroll
+        for (int v = 0; v < 5; v++) u[v] = o[v] + factor * f[v];
+        store5(a.var, i, u);
+        const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        store5(a.flux, i, z);
+        if (a.res) {
+            double r[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                r[v] = u[v] - o[v];
+                s += r[v] * r[v];
+                nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
+            }
+            store5(a.res, i, r);
+        }
+    }
+    if (a.part) {
+        const double t = block_sum(s);
+        if (threadIdx.x == 0) a.part[blockIdx.x] = t;
+    }
+    if (a.bad || a.badpart) {
+        const int t = block_sum_i(nb);
+        if (threadIdx.x == 0) {
+            if (a.badpart) a.badpart[blockIdx.x] = t;
+            else if (t) atomicAdd(a.bad, t);
+        }
+    }
 }
 
 // residual_kernel (validation.h:27-35); with 'part' set also the per-node
@@ -1339,47 +1452,6 @@ __global__ __launch_bounds__(kBlock) void k_min_final_seg(const int* __restrict_
     if (threadIdx.x == 0) {
         mindt[s] = m;
         if (m < 0.0 && sstatus[s] == 0) sstatus[s] = 1;
-    }
-}
-
-__global__ void k_step_factor_seg(const int4* __restrict__ blk, const double* __restrict__ vol,
-                                  const double* __restrict__ mindt, double* __restrict__ sf,
-                                  const int* __restrict__ halt)
-{
-    if (halted(halt)) return;
-    const int4 B = blk[blockIdx.x];
-    const int i = B.y + threadIdx.x;
-    if (i < B.z) sf[i] = mindt[B.x] / vol[i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_residual_seg(const int4* __restrict__ blk, const double* __restrict__ old,
-                                                         const double* __restrict__ var, double* __restrict__ res,
-                                                         double* __restrict__ part, int* __restrict__ badpart,
-                                                         const int* __restrict__ halt)
-{
-#pragma clang fp contract(off)
-    if (halted(halt)) return;
-    const int4 B = blk[blockIdx.x];
-    const int i = B.y + threadIdx.x;
-    double s = 0.0;
-    int nb = 0;
-    if (i < B.z) {
-        double o[5], u[5], r[5];
-        load5(old, i, o);
-        load5(var, i, u);
-#pragma unroll
-        for (int v = 0; v < 5; v++) {
-            r[v] = u[v] - o[v];
-            s += r[v] * r[v];
-            nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
-        }
-        store5(res, i, r);
-    }
-    const double t = block_sum(s);
-    const int tb = block_sum_i(nb);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = t;
-        badpart[blockIdx.x] = tb;
     }
 }
 

@@ -78,6 +78,7 @@ struct InvPlan {
     int* d_ptr = nullptr;        // n_targets + 1
     int2* d_inc2 = nullptr;      // dim-2 maps: {other endpoint, src | col << 31}
     int* d_inc1 = nullptr;       // dim-1 maps: src
+    int* d_nptr = nullptr;       // dim-1 maps: CSR pointer over every to-set element (d_ptr itself when dense)
     // blocked-gather plan (dim-2 maps onto a partitioned node set)
     int bk_blocks = 0;           // 0 = not available
     int bk_entries = 0;          // jagged-diagonal entries (= incidences)
@@ -892,6 +893,14 @@ static int build_inverse(op_map m)
     } else {
         HIPCHK(hipMalloc(&P.d_inc1, sizeof(int) * std::max<size_t>(1, inc1.size())));
         if (!inc1.empty()) HIPCHK(hipMemcpy(P.d_inc1, inc1.data(), sizeof(int) * inc1.size(), hipMemcpyHostToDevice));
+        if (P.dense) {
+            P.d_nptr = P.d_ptr;
+        } else {       // targets are in increasing element order, so the entries line up
+            std::vector<int> nptr(nt + 1, 0);
+            for (int t = 0; t < nt; t++) nptr[t + 1] = nptr[t] + cnt[t];
+            HIPCHK(hipMalloc(&P.d_nptr, sizeof(int) * nptr.size()));
+            HIPCHK(hipMemcpy(P.d_nptr, nptr.data(), sizeof(int) * nptr.size(), hipMemcpyHostToDevice));
+        }
     }
     P.built = true;
     return 0;
@@ -929,6 +938,7 @@ static int set_part_nodes(op_map m, int nodes)
 
 static void free_invplan(InvPlan& P)
 {
+    if (P.d_nptr && P.d_nptr != P.d_ptr) hipFree(P.d_nptr);
     for (void* p : {(void*)P.d_targets, (void*)P.d_ptr, (void*)P.d_inc2, (void*)P.d_inc1, (void*)P.d_bk_bnode,
                     (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase, (void*)P.d_bk_pbase,
                     (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w3})
@@ -1214,17 +1224,14 @@ static int bnd_targets(op_map b2n)
     return 0;
 }
 
-// kseg/tseg (batched solves): far-field constants per mesh and the mesh of
-// each boundary target; null for one mesh with the runtime's constants.
-static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat flux, const int* status,
-                      const Consts* kseg = nullptr, const int* tseg = nullptr)
+static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat flux, const int* status)
 {
     TRY(bnd_targets(b2n));
     const InvPlan& P = M(b2n)->inv;
     if (P.n_targets == 0) return 0;
     k_bnd_flux<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
                                                            iptr(group), dptr(bw), dptr(var), dptr(flux), R.k,
-                                                           status, kseg, tseg);
+                                                           status);
     return check_launch();
 }
 
