@@ -886,6 +886,16 @@ __device__ __forceinline__ double block_min(double x)
     return x;
 }
 
+// block_min with the result in every thread.
+__device__ __forceinline__ double block_min_all(double x)
+{
+    __shared__ double bc;
+    x = block_min(x);
+    if (threadIdx.x == 0) bc = x;
+    __syncthreads();
+    return bc;
+}
+
 __device__ __forceinline__ double block_sum(double x)
 {
 #pragma clang fp contract(off)
@@ -973,10 +983,9 @@ __global__ __launch_bounds__(kBlock) void k_min_part(int n, const double* __rest
 // set, a negative minimum raises abort code 1 (euler3d.cpp:480-484).
 __global__ __launch_bounds__(kBlock) void k_min_final(int np, const double* __restrict__ part,
                                                        double init, double* __restrict__ out,
-                                                       int* __restrict__ status, int* __restrict__ zero = nullptr)
+                                                       int* __restrict__ status)
 {
     if (status && *status != 0) return;
-    if (zero && threadIdx.x == 0) *zero = 0;   // a counter the step's later kernels accumulate into
     double m = DBL_MAX;
     for (int i = threadIdx.x; i < np; i += kBlock) m = (part[i] < m) ? part[i] : m;
     m = block_min(m);
@@ -1021,7 +1030,9 @@ __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double
 // One Runge-Kutta stage of a multigrid smoothing step after its edge flux,
 // fused into one pass over the nodes: the node's boundary and far-field
 // increments (flux.h:14-39, k_bnd_flux's arithmetic in boundary-element
-// order), on the first stage the step factor (time_stepping_kernels.h:43-64),
+// order), on the first stage the minimum dt (each workgroup reduces the
+// per-workgroup minima of k_calc_dt over its mesh, as k_min_final does,
+// euler3d.cpp:480-484) and the step factor (time_stepping_kernels.h:43-64),
 // the stage update (time_stepping_kernels.h:66-86) and on the last stage the
 // residual with its per-workgroup sum of squares and NaN/Inf count
 // (validation.h:27-44, 102-115).  Each node sees the same operations in the
@@ -1029,7 +1040,12 @@ __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double
 struct StageArgs {
     int n, rk;
     const int4* blk;           // batched solves: {segment, first, end} per workgroup; null: plain blocks over n
-    const double* dmin;        // minimum dt: [0], or per segment
+    const double* mpart;       // first stage: per-workgroup minima of k_calc_dt ...
+    int np;                    // ... [0, np) for one mesh, or
+    const int* sblk;           // ... [sblk[s], sblk[s + 1]) for segment s
+    double* mout;              // minimum dt written: [0], or per segment
+    int* mstatus;              // abort code 1 on a negative minimum: one status, or per segment
+    int* zero;                 // first stage: a counter the last stage accumulates into, cleared
     const double* vol;
     double* sf;
     double* flux;
@@ -1052,14 +1068,31 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 #pragma clang fp contract(off)
     if (halted(a.status)) return;
     int i, lim, seg = 0;
+    bool first;                // first workgroup of its mesh
     if (a.blk) {
         const int4 B = a.blk[blockIdx.x];
         seg = B.x;
         i = B.y + threadIdx.x;
         lim = B.z;
+        first = (int)blockIdx.x == a.sblk[seg];
     } else {
         i = blockIdx.x * blockDim.x + threadIdx.x;
         lim = a.n;
+        first = blockIdx.x == 0;
+    }
+    double mdt = 0.0;
+    if (a.rk == 0) {
+        const int lo = a.blk ? a.sblk[seg] : 0, hi = a.blk ? a.sblk[seg + 1] : a.np;
+        double m = DBL_MAX;
+        for (int j = lo + threadIdx.x; j < hi; j += kBlock) m = (a.mpart[j] < m) ? a.mpart[j] : m;
+        m = block_min_all(m);
+        mdt = (m < DBL_MAX) ? m : DBL_MAX;
+        if (first && threadIdx.x == 0) {
+            a.mout[seg] = mdt;
+            if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
+            if (a.zero) *a.zero = 0;
+        }
+        if (mdt < 0.0 && a.status) return;   // a halting context aborts before any update
     }
     double s = 0.0;
     int nb = 0;
@@ -1097,7 +1130,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
         }
         double sfi;
         if (a.rk == 0) {
-            sfi = a.dmin[seg] / a.vol[i];
+            sfi = mdt / a.vol[i];
             a.sf[i] = sfi;
         } else {
             sfi = a.sf[i];
@@ -1438,21 +1471,6 @@ __global__ __launch_bounds__(kBlock) void k_calc_dt_seg(const int4* __restrict__
     if (i < B.z) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
     const double m = block_min(d);
     if (threadIdx.x == 0) part[blockIdx.x] = m;
-}
-
-__global__ __launch_bounds__(kBlock) void k_min_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,
-                                                          double* __restrict__ mindt, int* __restrict__ sstatus,
-                                                          const int* __restrict__ halt)
-{
-    if (halted(halt)) return;
-    const int s = blockIdx.x;
-    double m = DBL_MAX;
-    for (int i = sblk[s] + threadIdx.x; i < sblk[s + 1]; i += kBlock) m = (part[i] < m) ? part[i] : m;
-    m = block_min(m);
-    if (threadIdx.x == 0) {
-        mindt[s] = m;
-        if (m < 0.0 && sstatus[s] == 0) sstatus[s] = 1;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,
