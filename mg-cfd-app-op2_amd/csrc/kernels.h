@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_blocked_pf(
     const int2* __restrict__ bnode, const int* __restrict__ hptr, const int* __restrict__ halo,
     const int* __restrict__ jbase, const int* __restrict__ pbase, const int* __restrict__ jpref,
     const int* __restrict__ nptr, const unsigned* __restrict__ jslot, const double* __restrict__ jw, int nent,
-    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status)
+    const double* __restrict__ var, double* __restrict__ flux, Consts k, const int* __restrict__ status, int fresh)
 {
     extern __shared__ double sm[];
     if (halted(status)) return;
@@ -311,7 +311,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_blocked_pf(
     double uo[5], uh[5], acc[5];
     if (own) {
         load5(var, nd, uo);
-        load5(flux, nd, acc);
+        if (fresh) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] = 0.0;
+        } else {
+            load5(flux, nd, acc);
+        }
     }
     if (hn >= 0) load5(var, hn, uh);
     if (own) stage_record<EXACT>(sm + t * kRec, uo);
@@ -538,13 +543,13 @@ __device__ __forceinline__ void part_load_edges(const PartInfo& P, const unsigne
 template <int T, int MAXJ, int FL>
 __device__ __forceinline__ void part_load_state(const PartInfo& P, const double* __restrict__ var,
                                                 const double* __restrict__ flux, int t, const int src[MAXJ],
-                                                double st[MAXJ], double fl[FL])
+                                                double st[MAXJ], double fl[FL], bool fresh)
 {
     const double* fo = flux + (size_t)P.n0 * 5;
 #pragma unroll
     for (int i = 0; i < MAXJ; i++) st[i] = var[src[i]];
 #pragma unroll
-    for (int i = 0; i < FL; i++) fl[i] = fo[min(t + i * T, P.cnt * 5 - 1)];
+    for (int i = 0; i < FL; i++) fl[i] = fresh ? 0.0 : fo[min(t + i * T, P.cnt * 5 - 1)];
 }
 
 template <typename R, int T, int MAXJ, int FL, int RS>
@@ -593,7 +598,7 @@ template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
 __global__ __launch_bounds__(T) void k_flux_part(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
-    double* __restrict__ flux, Consts k, const int* __restrict__ status)
+    double* __restrict__ flux, Consts k, const int* __restrict__ status, int fresh)
 {
     constexpr int RS = PartRec<R>::kStride;
     constexpr int FL = (5 * kPart + T - 1) / T;
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(T) void k_flux_part(
     part_halo_entries<T, MAXJ>(P, halo, t, hn);
     part_load_edges<R, T, EPT>(P, ecode, ew, nloc, t, code, w0, w1, w2);
     part_stage_select<T, MAXJ>(P, t, hn, src);
-    part_load_state<T, MAXJ, FL>(P, var, flux, t, src, st, fl);
+    part_load_state<T, MAXJ, FL>(P, var, flux, t, src, st, fl, fresh != 0);
     part_store_lds<R, T, MAXJ, FL, RS>(P, t, st, fl, rec, acc);
     __syncthreads();
     // 2./3. edges
@@ -675,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_gather(
     int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr,
     const int2* __restrict__ inc, const double4* __restrict__ wpk,
     const double* __restrict__ var, double* __restrict__ flux, Consts k,
-    const int* __restrict__ status)
+    const int* __restrict__ status, int fresh)
 {
     if (halted(status)) return;
     const int tid = xcd_block() * kBlock + threadIdx.x;
@@ -683,7 +688,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_gather(
     const int n = targets ? targets[tid] : tid;
     double U[5], acc[5];
     load5(var, n, U);
-    load5(flux, n, acc);
+    if (fresh) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v] = 0.0;
+    } else {
+        load5(flux, n, acc);
+    }
     Th o;
     thermo<EXACT>(U, o);
     const int beg = ptr[tid], end = ptr[tid + 1];
@@ -1060,6 +1070,7 @@ struct StageArgs {
     double* part;              // last stage, finest level: per-workgroup sum of squares
     int* bad;                  // ... NaN/Inf count: one counter (atomic) ...
     int* badpart;              // ... or per workgroup
+    int zero_flux;             // reset the flux to zero (else the next edge flux starts fresh)
     const int* status;
 };
 
@@ -1140,8 +1151,10 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 #pragma unroll
         for (int v = 0; v < 5; v++) u[v] = o[v] + factor * f[v];
         store5(a.var, i, u);
-        const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        store5(a.flux, i, z);
+        if (a.zero_flux) {
+            const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            store5(a.flux, i, z);
+        }
         if (a.res) {
             double r[5];
 #pragma unroll

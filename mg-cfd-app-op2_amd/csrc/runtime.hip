@@ -1107,29 +1107,31 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
     return 0;
 }
 
-static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat flux, const int* status)
+static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat flux, const int* status, int fresh)
 {
     if (exact)
         k_flux_blocked_pf<true, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
             P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
-            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status);
+            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     else
         k_flux_blocked_pf<false, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
             P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
-            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status);
+            P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     return check_launch();
 }
 
 template <typename Real, int T, int EPT, int MAXJ>
 static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                         const int* status)
+                         const int* status, int fresh)
 {
     if (coloured)
         k_flux_part<Real, T, EPT, MAXJ, true><<<P.pt_parts, T, lds, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
+            fresh);
     else
         k_flux_part<Real, T, EPT, MAXJ, false><<<P.pt_parts, T, lds, R.stream>>>(
-            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status);
+            P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
+            fresh);
     return check_launch();
 }
 
@@ -1137,13 +1139,13 @@ static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t 
 // largest part allows it.
 template <typename Real, int T, int EPT>
 static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                         const int* status)
+                         const int* status, int fresh)
 {
     constexpr int kSmall = 3072 / T, kLarge = 4096 / T;
     static_assert(kPartSlots * 5 <= kLarge * T, "staged entries exceed the kernel's registers");
     if (P.pt_rec_cap * 5 <= kSmall * T)
-        return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status);
-    return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status);
+        return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status, fresh);
+    return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status, fresh);
 }
 
 // Real = double: the FP64 part kernel, 256 threads per part; Real = float:
@@ -1151,14 +1153,14 @@ static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t 
 // waves) share one part, which shortens each workgroup's edge phase.
 template <typename Real>
 static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                       const int* status)
+                       const int* status, int fresh)
 {
     constexpr int T = sizeof(Real) == 4 ? 512 : 256;
     const int ept = (P.pt_max_ne + T - 1) / T;
-    if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status);
-    if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status);
+    if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status, fresh);
+    if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status, fresh);
+    if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status, fresh);
+    if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status, fresh);
     return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 3072);
 }
 
@@ -1175,7 +1177,11 @@ static int prepare_flux(int variant, op_map map, op_dat w)
     return 0;
 }
 
-static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flux, const int* status)
+// fresh: the edge fluxes start from zero instead of adding to the flux dat
+// (the device MG driver's later Runge-Kutta stages, whose flux the previous
+// stage left to be overwritten); the HBM-atomic variant always adds.
+static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flux, const int* status,
+                       bool fresh = false)
 {
     if (variant == MGCFD_FLUX_ATOMIC) {
         const double4* wp = nullptr;
@@ -1194,19 +1200,20 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
         if (P.pt_parts == 0) return fail("edge-flux variant %d needs an edge map onto every node of its set", variant);
         if (mixed_variant(variant))
             return launch_part<float>(variant == MGCFD_FLUX_MIXED_COLOURED, P, P.d_pt_w32, P.pt_lds32, var, flux,
-                                      status);
-        return launch_part<double>(variant == MGCFD_FLUX_PART_COLOURED, P, P.d_pt_w, P.pt_lds, var, flux, status);
+                                      status, fresh);
+        return launch_part<double>(variant == MGCFD_FLUX_PART_COLOURED, P, P.d_pt_w, P.pt_lds, var, flux, status,
+                                   fresh);
     }
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0)
-        return launch_blocked_pf(variant == MGCFD_FLUX_GATHER_EXACT, P, var, flux, status);
+        return launch_blocked_pf(variant == MGCFD_FLUX_GATHER_EXACT, P, var, flux, status, fresh);
     const double4* wp = nullptr;
     TRY(packed_weights(w, &wp));
     if (variant == MGCFD_FLUX_GATHER_FAST)
         k_flux_gather<false><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
-            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status);
+            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status, fresh);
     else
         k_flux_gather<true><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
-            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status);
+            P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status, fresh);
     return check_launch();
 }
 
