@@ -26,6 +26,12 @@ also launches:
   rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
   rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
   python3 bench.py --summarize-pmc OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
+
+Beside the headline the JSON line carries: the unstructured-stream loop on
+the same copies (the access pattern's ceiling, which the reference measures
+beside the flux under -b), the flux at a 1M-node shard shape, batched solves
+(64 meshes), served solves (Poisson arrivals) and, with --large, one
+8M-node 50-cycle solve in mixed precision and in FP64.
 """
 import argparse
 import csv
@@ -88,6 +94,67 @@ def time_variant(rt, solvers, variant, iters, warmup):
         mgcfd._check(rt.L.mgcfd_time_flux(variant, n, maps, vs, ws, fs, warmup, C.byref(ms)), "warmup")
     mgcfd._check(rt.L.mgcfd_time_flux(variant, n, maps, vs, ws, fs, iters, C.byref(ms)), "mgcfd_time_flux")
     return ms.value
+
+
+def time_stream(rt, solvers, iters, warmup):
+    """The unstructured-stream loop (unstructured_stream.h:7-57): the flux
+    loop's access pattern with almost no arithmetic, timed like the flux."""
+    n = len(solvers)
+    maps = (mgcfd.OpMap * n)(*[s.lv[0].edge_to_node for s in solvers])
+    vs = (mgcfd.OpDat * n)(*[s.lv[0].variables for s in solvers])
+    ws = (mgcfd.OpDat * n)(*[s.lv[0].edge_weights for s in solvers])
+    fs = (mgcfd.OpDat * n)(*[s.lv[0].fluxes for s in solvers])
+    ms = C.c_double(0)
+    if warmup:
+        mgcfd._check(rt.L.mgcfd_time_stream(n, maps, vs, ws, fs, warmup, C.byref(ms)), "warmup")
+    mgcfd._check(rt.L.mgcfd_time_stream(n, maps, vs, ws, fs, iters, C.byref(ms)), "mgcfd_time_stream")
+    return ms.value
+
+
+def bench_shard(rt, variant, steps, warmup, n=1_000_000, e=3_100_000, copies=3):
+    """The edge flux at one GPU's share of an 8M-node solve split over 8 GPUs:
+    a 1M-node / 3.1M-edge mesh, three copies rotated (beyond the 256 MB
+    cache)."""
+    mesh = mgcfd.mesh_knn(n, e, 1, seed=7)[0]
+    sols = [mgcfd.Solver(rt, [mesh], MACH, AOA) for _ in range(copies)]
+    for k, s in enumerate(sols):
+        s.setup()
+        rt.upload(s.lv[0].variables, perturbed(n, s.fs["ff_variable"], seed=200 + k))
+    rt.sync()
+    ms = time_variant(rt, sols, mgcfd.FLUX_VARIANTS[variant], steps, warmup)
+    nb = algorithmic_bytes(n, e)
+    out = dict(nodes=n, edges=e, variant=variant, us_rotating=round(ms * 1e3, 2),
+               gedges=round(e / (ms * 1e-3) / 1e9, 2), hbm_frac=round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+    log(f"bench: shard: {out}")
+    return out
+
+
+def bench_large(rt, cycles=50, n=8_000_000, mach=0.7, aoa=2.0):
+    """One large solve (8M level-0 nodes, E = 3.1 N, 4 levels, M0.7 a2deg):
+    'cycles' cycles in mixed precision against the same cycles in plain FP64,
+    each after one warm-up cycle that captures its graph."""
+    t = time.perf_counter()
+    mesh = mgcfd.mesh_knn(n, int(round(3.1 * n)), 4, seed=8)
+    log(f"bench: large: generated {[len(l['coords']) for l in mesh]} nodes per level, {time.perf_counter() - t:.1f} s")
+    out = dict(level_nodes=[len(l["coords"]) for l in mesh], level_edges=[len(l["edge_to_node"]) for l in mesh],
+               cycles=cycles)
+    ref = None
+    for variant in ("gather_exact", "mixed_atomic"):
+        b = mgcfd.Batch(rt, [mesh], [mach], [aoa], variant)
+        b.run(1)
+        t0 = time.perf_counter()
+        st, _ = b.run(cycles)
+        el = time.perf_counter() - t0
+        u = b.fetch(0, 0)
+        b.close()
+        out[variant] = dict(seconds=round(el, 3), ms_per_cycle=round(el / cycles * 1e3, 3), status_ok=bool((st == 0).all()))
+        if ref is None:
+            ref = u
+        else:
+            out[variant]["max_rel_err_vs_fp64"] = float(f"{(np.abs(u - ref).max(axis=0) / np.abs(ref).max(axis=0)).max():.3e}")
+        log(f"bench: large: {variant}: {out[variant]}")
+    out["speedup"] = round(out["gather_exact"]["seconds"] / out["mixed_atomic"]["seconds"], 3)
+    return out
 
 
 def check_variant(rt, solver, mesh0, variant, orc):
@@ -310,6 +377,8 @@ def main():
     ap.add_argument("--batch-cycles", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--no-shard", action="store_true", help="skip the 1M-node shard-shape flux measurement")
+    ap.add_argument("--large", action="store_true", help="also run the 8M-node, 50-cycle solve (about 2 minutes)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
     args = ap.parse_args()
@@ -365,6 +434,13 @@ def main():
     best = min(eligible, key=lambda k: eligible[k]["ms_rotating"])
     r = results[best]
     ms = r["ms_rotating"]
+    s_rot = time_stream(rt, solvers, args.steps, args.warmup)
+    s_res = time_stream(rt, solvers[:1], args.steps, args.warmup)
+    stream = dict(ms_rotating=round(s_rot, 6), ms_resident=round(s_res, 6),
+                  flux_over_stream=round(ms / s_rot, 3), flux_over_stream_gather_exact=round(
+                      results["gather_exact"]["ms_rotating"] / s_rot, 3))
+    log(f"bench: unstructured stream: {s_rot * 1e3:.2f} us/call rotating, {s_res * 1e3:.2f} us/call resident; "
+        f"flux/stream {stream['flux_over_stream']} ({best})")
     achieved = nb / (ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
@@ -376,8 +452,10 @@ def main():
         var = rt.fetch(solvers[0].lv[0].variables)
         w = rt.fetch(solvers[0].lv[0].edge_weights)
         cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds)
+    shard = None if args.no_shard else bench_shard(rt, best, args.steps, args.warmup)
     batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles)
     serving = None if args.no_serve else bench_serving(rt, args.serve_requests, args.serve_rate)
+    large = bench_large(rt) if args.large else None
     rt.close()
 
     out = {
@@ -401,8 +479,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "stream": stream,
+        "shard": shard,
         "batched": batched,
         "serving": serving,
+        "large": large,
         "variants": {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in r2.items()}
                      for k, r2 in results.items()},
     }

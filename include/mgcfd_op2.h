@@ -274,6 +274,13 @@ int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
 int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
                     const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
 
+/* The same timing for the unstructured-stream loop (unstructured_stream.h:7-57,
+ * which the reference runs beside the flux under -b, euler3d.cpp:518-523):
+ * the flux loop's gathers and increments with almost no arithmetic, the
+ * access pattern's own ceiling.  'fluxes' receive the stream's dummy sums. */
+int mgcfd_time_stream(int n_copies, const op_map* edge_maps, const op_dat* vars, const op_dat* weights,
+                      const op_dat* fluxes, int iters, double* ms_per_call);
+
 /*
  * Batched solves (no counterpart in the reference, which runs one mesh per
  * process): n independent multigrid meshes, all with the same number of
