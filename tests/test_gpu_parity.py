@@ -9,6 +9,8 @@ held to the relative tolerances written next to each assertion.
 Relative error of an [N][5] array per variable v:
     max_i |gpu[i,v] - oracle[i,v]| / max_i |oracle[i,v]|.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -152,6 +154,30 @@ def test_stream_bitwise(rt, orc, knn_small):
     var = perturbed_state(len(m0["coords"]), c.ff_variable[:], seed=31)
     rt.upload(s.lv[0].variables, var)
     s.stream(0)
+    o = np.zeros_like(var)
+    orc.unstructured_stream(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o)
+    assert bitwise(s.state(0, "dummy_fluxes"), o)
+
+
+def test_time_stream_entry_matches_stream_loop(rt, orc, knn_small):
+    """mgcfd_time_stream (the bench's timing of the unstructured-stream loop)
+    launches the same kernel: one call from zero dummy fluxes equals the
+    oracle's unstructured_stream bitwise."""
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, knn_small, 0.5, 3.0, measure_mem_bound=True)
+    s.setup()
+    ref = oracle_setup(orc, knn_small, c)
+    m0 = knn_small[0]
+    n = len(m0["coords"])
+    var = perturbed_state(n, c.ff_variable[:], seed=33)
+    h = s.lv[0]
+    rt.upload(h.variables, var)
+    rt.upload(h.dummy_fluxes, np.zeros((n, 5)))
+    ms = C.c_double(0)
+    rc = rt.L.mgcfd_time_stream(1, (mgcfd.OpMap * 1)(h.edge_to_node), (mgcfd.OpDat * 1)(h.variables),
+                                (mgcfd.OpDat * 1)(h.edge_weights), (mgcfd.OpDat * 1)(h.dummy_fluxes), 1,
+                                C.byref(ms))
+    assert rc == 0 and ms.value > 0.0
     o = np.zeros_like(var)
     orc.unstructured_stream(np.ascontiguousarray(m0["edge_to_node"], np.int32), var, ref[0]["ew"], o)
     assert bitwise(s.state(0, "dummy_fluxes"), o)
@@ -307,6 +333,34 @@ def test_solve_parity(rt, orc, meshname, mode, request):
     for l in range(len(mesh)):
         assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
     assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("variant", ["gather_exact", "part_atomic", "mixed_atomic"])
+def test_device_run_adds_to_initial_flux(rt, orc, box8, variant):
+    """A flux dat that is not zero when the run starts: each level's first
+    Runge-Kutta stage adds its edge fluxes to it, as the reference's INC loop
+    does, and the later stages start from the zeros the time step leaves
+    (the device driver starts them fresh without reading the dat)."""
+    rng = np.random.default_rng(11)
+    seed_flux = [1e-3 * rng.standard_normal((len(m["coords"]), 5)) for m in box8]
+
+    def seed(arrs):
+        for a, x in zip(arrs, seed_flux):
+            a["flux"][:] = x
+    ref = orc.solve(box8, 2, 0.5, 3.0, after_setup=seed)
+    s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.setup()
+    for l, x in enumerate(seed_flux):
+        rt.upload(s.lv[l].fluxes, x)
+    st, rms = s.run_device(2, use_graph=True)
+    assert st == 0 and ref["status"] == 0
+    tol = MIXED_STATE_TOL if variant.startswith("mixed") else STATE_TOL
+    for l in range(len(box8)):
+        got = s.state(l)
+        exp = ref["levels"][l]["variables"]
+        assert rel_err(got, exp) <= tol, f"level {l}"
+        assert not s.state(l, "fluxes").any(), f"level {l}: flux not zeroed by the last stage"
 
 
 def test_graph_equals_eager_and_is_deterministic(rt, knn_small):
