@@ -944,14 +944,12 @@ __device__ __forceinline__ double min_candidate(double d) { return (d < DBL_MAX)
 // calculate_dt_kernel (time_stepping_kernels.h:13-32).  With 'old' non-null
 // it also performs copy_double_kernel (copy_double_kernel.h:6-13) and with
 // 'part' non-null it emits the per-workgroup min for get_min_dt_kernel.
-// dt of node i (time_stepping_kernels.h:13-32), optionally saving old = var.
-__device__ __forceinline__ double calc_dt_node(int i, const double* __restrict__ var,
-                                               const double* __restrict__ vol, double* __restrict__ dt,
-                                               double* __restrict__ old)
+// dt of node i with state U (time_stepping_kernels.h:13-32), optionally
+// saving old = U.
+__device__ __forceinline__ double calc_dt_state(int i, const double U[5], const double* __restrict__ vol,
+                                                double* __restrict__ dt, double* __restrict__ old)
 {
 #pragma clang fp contract(off)
-    double U[5];
-    load5(var, i, U);
     if (old) store5(old, i, U);
     const double rho = U[0], E = U[4];
     const double ux = U[1] / rho, uy = U[2] / rho, uz = U[3] / rho;
@@ -962,6 +960,29 @@ __device__ __forceinline__ double calc_dt_node(int i, const double* __restrict__
     dt[i] = v;
     return v;
 }
+
+// dt of node i (time_stepping_kernels.h:13-32), optionally saving old = var.
+__device__ __forceinline__ double calc_dt_node(int i, const double* __restrict__ var,
+                                               const double* __restrict__ vol, double* __restrict__ dt,
+                                               double* __restrict__ old)
+{
+    double U[5];
+    load5(var, i, U);
+    return calc_dt_state(i, U, vol, dt, old);
+}
+
+// The next smoothing step's calculate_dt + copy (time_stepping_kernels.h:13-32,
+// copy_double_kernel.h:6-13) on the level a restriction or prolongation has
+// just produced, done by that kernel's thread for its node: dt, old = var and
+// the per-workgroup minimum, over plain blocks or a segment block table
+// ({segment, first, end} per workgroup, batched solves).
+struct DtArgs {
+    const int4* blk;
+    const double* vol;
+    double* dt;
+    double* old;
+    double* part;
+};
 
 __global__ __launch_bounds__(kBlock) void k_calc_dt(
     int n, const double* __restrict__ var, const double* __restrict__ vol,
@@ -1391,6 +1412,53 @@ __global__ void k_up_post(int nc, double* __restrict__ vc, const int* __restrict
 }
 
 // down_kernel (prolongation).
+// up_pre + up + up_post (mg.h:29-64) as one gather per coarse node of a
+// restriction whose every coarse node is a target (dense plan), followed by
+// the coarse level's calculate_dt.
+__global__ __launch_bounds__(kBlock) void k_up_gather_dt(
+    int nc, const int* __restrict__ ptr, const int* __restrict__ child, const double* __restrict__ vf,
+    double* __restrict__ vc, int* __restrict__ scratch, const int* __restrict__ status, DtArgs d)
+{
+#pragma clang fp contract(off)
+    if (halted(status)) return;
+    int c, lim;
+    if (d.blk) {
+        const int4 B = d.blk[blockIdx.x];
+        c = B.y + threadIdx.x;
+        lim = B.z;
+    } else {
+        c = blockIdx.x * blockDim.x + threadIdx.x;
+        lim = nc;
+    }
+    double m = DBL_MAX;
+    if (c < lim) {
+        const int beg = ptr[c], end = ptr[c + 1];
+        double acc[5];
+        if (beg == end) {
+            load5(vc, c, acc);             // no children: stale value kept (mg.h:58)
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] = 0.0;
+            int cnt = 0;
+            for (int q = beg; q < end; q++) {
+                double x[5];
+                load5(vf, child[q], x);
+#pragma unroll
+                for (int v = 0; v < 5; v++) acc[v] += x[v];
+                cnt += 1;
+            }
+            const double avg = 1.0 / (double)cnt;
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] *= avg;
+            store5(vc, c, acc);
+            scratch[c] = cnt;
+        }
+        m = min_candidate(calc_dt_state(c, acc, d.vol, d.dt, d.old));
+    }
+    m = block_min(m);
+    if (threadIdx.x == 0) d.part[blockIdx.x] = m;
+}
+
 __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__ vf,
                        const double* __restrict__ rf, const double* __restrict__ xf,
                        const double* __restrict__ rc, const double* __restrict__ xc,
@@ -1415,6 +1483,46 @@ __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__
     u[3] -= dz * (ra[3] - r[3]);
     u[4] -= dm * (ra[4] - r[4]);
     store5(vf, i, u);
+}
+
+// down_kernel (mg.h:66-88) followed by the fine level's calculate_dt.
+__global__ __launch_bounds__(kBlock) void k_down_dt(
+    int nf, const int* __restrict__ f2c, double* __restrict__ vf, const double* __restrict__ rf,
+    const double* __restrict__ xf, const double* __restrict__ rc, const double* __restrict__ xc,
+    const int* __restrict__ status, DtArgs d)
+{
+#pragma clang fp contract(off)
+    if (halted(status)) return;
+    int i, lim;
+    if (d.blk) {
+        const int4 B = d.blk[blockIdx.x];
+        i = B.y + threadIdx.x;
+        lim = B.z;
+    } else {
+        i = blockIdx.x * blockDim.x + threadIdx.x;
+        lim = nf;
+    }
+    double m = DBL_MAX;
+    if (i < lim) {
+        const int c = f2c[i];
+        const double dx = fabs(xf[3 * (size_t)i] - xc[3 * (size_t)c]);
+        const double dy = fabs(xf[3 * (size_t)i + 1] - xc[3 * (size_t)c + 1]);
+        const double dz = fabs(xf[3 * (size_t)i + 2] - xc[3 * (size_t)c + 2]);
+        const double dm = sqrt(dx * dx + dy * dy + dz * dz);
+        double u[5], r[5], ra[5];
+        load5(vf, i, u);
+        load5(rf, i, r);
+        load5(rc, c, ra);
+        u[0] -= dm * (ra[0] - r[0]);
+        u[1] -= dx * (ra[1] - r[1]);
+        u[2] -= dy * (ra[2] - r[2]);
+        u[3] -= dz * (ra[3] - r[3]);
+        u[4] -= dm * (ra[4] - r[4]);
+        store5(vf, i, u);
+        m = min_candidate(calc_dt_state(i, u, d.vol, d.dt, d.old));
+    }
+    m = block_min(m);
+    if (threadIdx.x == 0) d.part[blockIdx.x] = m;
 }
 
 // ------------------------------------------------------------------------
