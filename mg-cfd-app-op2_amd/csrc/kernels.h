@@ -14,7 +14,8 @@
 // Kernels marked "exact" disable FMA contraction and keep the reference's
 // operation order and true divisions, so they are bitwise equal to the
 // sequential CPU evaluation.  "fast" kernels use one reciprocal of the
-// density and FMA contraction.
+// density, refined hardware reciprocal and square-root estimates, and FMA
+// contraction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -82,6 +83,34 @@ __device__ __forceinline__ void flux_tensor(Th& t)
     else flux_tensor_fast(t);
 }
 
+// Reciprocal and square root of the fast kernels (not the correctly rounded
+// library expansions).  FP64: the hardware estimates
+// refined by Newton-Raphson (1/x: e = 1 - x r, r += r e, twice) and by one
+// Goldschmidt step plus one Newton correction (sqrt x from y = 1/sqrt x:
+// g = x y, h = y / 2, r = 1/2 - g h, g += g r, h += h r, g += (x - g g) h),
+// a few ulp at most, and exact at x = 0.  FP32: the hardware instructions
+// (1 ulp), well inside the mixed-precision arithmetic's own error.
+__device__ __forceinline__ double part_rcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double part_sqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, x), h, g);
+    return x > 0.0 ? g : 0.0;
+}
+__device__ __forceinline__ float part_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float part_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Reference order: u = m / rho, |u|^2, p, c = sqrt(gamma p / rho).
 __device__ __forceinline__ void thermo_exact(const double* U, Th& t)
 {
@@ -100,14 +129,14 @@ __device__ __forceinline__ void thermo_exact(const double* U, Th& t)
 __device__ __forceinline__ void thermo_fast(const double* U, Th& t)
 {
     t.rho = U[0]; t.mx = U[1]; t.my = U[2]; t.mz = U[3]; t.E = U[4];
-    const double inv = 1.0 / t.rho;
+    const double inv = part_rcp(t.rho);
     t.ux = t.mx * inv;
     t.uy = t.my * inv;
     t.uz = t.mz * inv;
     const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
-    t.speed = sqrt(ssq);
+    t.speed = part_sqrt(ssq);
     t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
-    t.c = sqrt(kGamma * t.p * inv);
+    t.c = part_sqrt(kGamma * t.p * inv);
     flux_tensor_fast(t);
 }
 
@@ -392,11 +421,11 @@ __device__ __forceinline__ PState<double> part_state(const double* r)
     const double2 a = q[0], b = q[1], c = q[2];
     PState<double> t;
     t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
-    const double inv = 1.0 / t.rho;
+    const double inv = part_rcp(t.rho);
     t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
     const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
     t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
-    t.s = sqrt(ssq) + sqrt(kGamma * t.p * inv);
+    t.s = part_sqrt(ssq) + part_sqrt(kGamma * t.p * inv);
     return t;
 }
 
@@ -406,11 +435,11 @@ __device__ __forceinline__ PState<float> part_state32(const float* r)
     const float2 a = q[0], b = q[1], c = q[2];
     PState<float> t;
     t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
-    const float inv = 1.0f / t.rho;
+    const float inv = part_rcp(t.rho);
     t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
     const float ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
     t.p = (float)(kGamma - 1.0) * (t.E - 0.5f * t.rho * ssq);
-    t.s = sqrtf(ssq) + sqrtf((float)kGamma * t.p * inv);
+    t.s = part_sqrt(ssq) + part_sqrt((float)kGamma * t.p * inv);
     return t;
 }
 
@@ -420,7 +449,7 @@ template <typename R, typename O>
 __device__ __forceinline__ void part_increment(const PState<R>& a, const PState<R>& b, R w0, R w1, R w2, R sc,
                                                O inc[5])
 {
-    const R ewt = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    const R ewt = part_sqrt(w0 * w0 + w1 * w1 + w2 * w2);
     const R factor = -ewt * sc * R(0.5) * (a.s + b.s);
     const R fx = R(-0.5) * w0, fy = R(-0.5) * w1, fz = R(-0.5) * w2;
     const R dea = a.E + a.p, deb = b.E + b.p;
