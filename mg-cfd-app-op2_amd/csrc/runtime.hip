@@ -396,30 +396,94 @@ static int build_map_device(op_map m)
 // uploads their maps and pending dats.  Sets, maps and dats may be declared
 // at any time (batched and served solves add meshes after the first loop);
 // an order, once chosen, never changes.
+template <typename F>
+static void parallel_for(int n, F&& f)
+{
+    const int nth = std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 64 || nth <= 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; t++)
+        pool.emplace_back([&] {
+            for (;;) {
+                const int i0 = next.fetch_add(8);
+                if (i0 >= n) break;
+                for (int i = i0; i < std::min(n, i0 + 8); i++) f(i);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+// n independent host tasks (n small: the levels of a mesh, the sets to
+// order), one per thread; returns the first failing task's code and keeps
+// its message.  Tasks may launch on the current stream and allocate.
+template <typename F>
+static int run_tasks(int n, F&& f)
+{
+    if (n <= 1) return n == 1 ? f(0) : 0;
+    std::vector<int> rc(n, 0);
+    std::vector<std::string> msg(n);
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    const int nth = std::min(n, 16);
+    for (int t = 0; t < nth; t++)
+        pool.emplace_back([&] {
+            for (int i; (i = next.fetch_add(1)) < n;) {
+                rc[i] = f(i);
+                if (rc[i]) msg[i] = g_err;       // the worker's own message
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (int i = 0; i < n; i++)
+        if (rc[i]) {
+            g_err = msg[i];
+            return rc[i];
+        }
+    return 0;
+}
+
 static int freeze()
 {
     if (R.frozen && !R.dirty) return 0;
-    // node-like sets: targets of dim-2 maps
-    for (auto& s : R.sets) {
-        if (!s.impl || S(&s)->frozen) continue;
+    // node-like sets: targets of dim-2 maps (each set on its own thread)
+    struct NodeSet {
+        op_set s;
         std::vector<const MapImpl*> ems;
         std::vector<int> fsz;
+    };
+    std::vector<NodeSet> ns;
+    for (auto& s : R.sets) {
+        if (!s.impl || S(&s)->frozen) continue;
+        NodeSet x{&s, {}, {}};
         for (auto& m : R.maps)
-            if (m.impl && m.to == &s && m.dim == 2 && m.from != &s) { ems.push_back(M(&m)); fsz.push_back(m.from->size); }
-        if (ems.empty() || s.size == 0) continue;
-        SetImpl* si = S(&s);
+            if (m.impl && m.to == &s && m.dim == 2 && m.from != &s) {
+                x.ems.push_back(M(&m));
+                x.fsz.push_back(m.from->size);
+            }
+        if (x.ems.empty() || s.size == 0) continue;
+        ns.push_back(std::move(x));
+    }
+    if (int rc = run_tasks((int)ns.size(), [&](int k) -> int {
+        op_set s = ns[k].s;
+        SetImpl* si = S(s);
         {
-            HostTimer tm("partition", s.name);
-            si->perm = partition_order(s.size, ems, fsz, si->seg, si->parts, si->geo);
+            HostTimer tm("partition", s->name);
+            si->perm = partition_order(s->size, ns[k].ems, ns[k].fsz, si->seg, si->parts, si->geo);
             si->geo.clear();
             si->geo.shrink_to_fit();
         }
-        si->iperm.assign(s.size, -1);
-        for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
-        for (int i = 0; i < s.size; i++)
-            if (si->iperm[i] < 0) return fail("internal error: node order of set %s is not a permutation", s.name);
-    }
+        si->iperm.assign(s->size, -1);
+        for (int i = 0; i < s->size; i++) si->iperm[si->perm[i]] = i;
+        for (int i = 0; i < s->size; i++)
+            if (si->iperm[i] < 0) return fail("internal error: node order of set %s is not a permutation", s->name);
+        return 0;
+    }))
+        return rc;
     // element sets mapped into a renumbered set: order by internal targets
+    std::vector<std::pair<op_set, op_map>> es;
     for (auto& s : R.sets) {
         if (!s.impl) continue;
         SetImpl* si = S(&s);
@@ -428,7 +492,12 @@ static int freeze()
         for (auto& m : R.maps)
             if (m.impl && m.from == &s && m.to->impl && !S(m.to)->perm.empty() && (key == nullptr || m.dim > key->dim))
                 key = &m;
-        if (!key) continue;
+        if (key) es.emplace_back(&s, key);
+    }
+    if (int rc = run_tasks((int)es.size(), [&](int k) -> int {
+        op_set_core& s = *es[k].first;
+        op_map key = es[k].second;
+        SetImpl* si = S(&s);
         HostTimer tm("order", s.name);
         const MapImpl* km = M(key);
         const SetImpl* ts = S(key->to);
@@ -459,7 +528,9 @@ static int freeze()
         counting(lo, tmp, si->perm);
         si->iperm.assign(s.size, 0);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
-    }
+        return 0;
+    }))
+        return rc;
     for (auto& s : R.sets)
         if (s.impl) S(&s)->frozen = true;
     R.frozen = true;
@@ -515,26 +586,6 @@ constexpr int kHaloCap = kPart;
 
 // Runs f(i) for i in [0, n) on up to 16 host threads (the plan builders'
 // per-part work is independent); f must be thread-safe.
-template <typename F>
-static void parallel_for(int n, F&& f)
-{
-    const int nth = std::min(16, (int)std::max(1u, std::thread::hardware_concurrency()));
-    if (n < 64 || nth <= 1) {
-        for (int i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::atomic<int> next{0};
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nth; t++)
-        pool.emplace_back([&] {
-            for (;;) {
-                const int i0 = next.fetch_add(8);
-                if (i0 >= n) break;
-                for (int i = i0; i < std::min(n, i0 + 8); i++) f(i);
-            }
-        });
-    for (auto& th : pool) th.join();
-}
 
 // Blocked-gather plan of one initial part (split while its halo exceeds
 // kHaloCap); offsets are local to the part and fixed up on concatenation.
