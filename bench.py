@@ -23,8 +23,8 @@ HBM traffic per launch comes from two rocprofv3 counter runs of this script
 for the plain copy kernel (k_stream_copy) of known size that this script
 also launches:
 
-  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
-  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu
+  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard
+  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard
   python3 bench.py --summarize-pmc OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
 
 Beside the headline the JSON line carries: the unstructured-stream loop on
