@@ -23,13 +23,14 @@ HBM traffic per launch comes from two rocprofv3 counter runs of this script
 for the plain copy kernel (k_stream_copy) of known size that this script
 also launches:
 
-  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard
-  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard
+  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard --no-solve
+  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard --no-solve
   python3 bench.py --summarize-pmc OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
 
 Beside the headline the JSON line carries: the unstructured-stream loop on
 the same copies (the access pattern's ceiling, which the reference measures
-beside the flux under -b), the flux at a 1M-node shard shape, batched solves
+beside the flux under -b), the flux at a 1M-node shard shape, a 50-cycle
+multigrid solve of the 300K-node mesh (4 levels) per flux variant, batched solves
 (64 meshes), served solves (Poisson arrivals) and, with --large, one
 8M-node 50-cycle solve in mixed precision and in FP64.
 """
@@ -127,6 +128,46 @@ def bench_shard(rt, variant, steps, warmup, n=1_000_000, e=3_100_000, copies=3):
                gedges=round(e / (ms * 1e-3) / 1e9, 2), hbm_frac=round(nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
     log(f"bench: shard: {out}")
     return out
+
+
+def bench_solve(rt, cycles=50):
+    """Whole multigrid solve on the 300K-node mesh with its 4 levels (the
+    euler3d.cpp -g N loop, device-resident, one captured graph per cycle):
+    'cycles' cycles after one warm-up cycle, per flux variant, with each
+    state's largest per-variable difference from the FP64 exact one."""
+    mesh = mgcfd.mesh_knn(N_NODES, N_EDGES, 4, seed=2)
+    out = dict(level_nodes=[len(l["coords"]) for l in mesh], cycles=cycles)
+    ref = None
+    for variant in ("gather_exact", "part_atomic", "mixed_atomic"):
+        b = mgcfd.Batch(rt, [mesh], [MACH], [AOA], variant)
+        b.run(1)
+        t0 = time.perf_counter()
+        st, _ = b.run(cycles)
+        el = time.perf_counter() - t0
+        u = b.fetch(0, 0)
+        b.close()
+        out[variant] = dict(ms_per_cycle=round(el / cycles * 1e3, 4), status_ok=bool((st == 0).all()))
+        if ref is None:
+            ref = u
+        else:
+            out[variant]["max_rel_err_vs_gather_exact"] = float(
+                f"{(np.abs(u - ref).max(axis=0) / np.abs(ref).max(axis=0)).max():.3e}")
+    log(f"bench: solve: {out}")
+    return out, mesh
+
+
+def cpu_solve_cycle(orc, mesh):
+    """One multigrid cycle of the same 4-level mesh in the C oracle (the
+    reference's loops restated, one thread): a 1-cycle solve minus a
+    0-cycle one (the setup loops)."""
+    t = time.perf_counter()
+    orc.solve(mesh, 0, MACH, AOA)
+    t0 = time.perf_counter() - t
+    t = time.perf_counter()
+    orc.solve(mesh, 1, MACH, AOA)
+    t1 = time.perf_counter() - t
+    return dict(ms_per_cycle=round((t1 - t0) * 1e3, 1), kind="port", cores=1,
+                sample="one cycle of the 300K-node 4-level solve in the C oracle (setup subtracted), one thread")
 
 
 def bench_large(rt, cycles=50, n=8_000_000, mach=0.7, aoa=2.0):
@@ -378,6 +419,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--no-shard", action="store_true", help="skip the 1M-node shard-shape flux measurement")
+    ap.add_argument("--no-solve", action="store_true", help="skip the 300K-node 50-cycle solve timing")
     ap.add_argument("--large", action="store_true", help="also run the 8M-node, 50-cycle solve (about 2 minutes)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
@@ -453,6 +495,13 @@ def main():
         w = rt.fetch(solvers[0].lv[0].edge_weights)
         cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds)
     shard = None if args.no_shard else bench_shard(rt, best, args.steps, args.warmup)
+    solve = None
+    if not args.no_solve:
+        solve, solve_mesh = bench_solve(rt)
+        if cpu is not None:
+            cpu["solve"] = cpu_solve_cycle(orc, solve_mesh)
+            log(f"bench: cpu solve: {cpu['solve']}")
+        del solve_mesh
     batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles)
     serving = None if args.no_serve else bench_serving(rt, args.serve_requests, args.serve_rate)
     large = bench_large(rt) if args.large else None
@@ -481,6 +530,7 @@ def main():
         "cpu_baseline": cpu,
         "stream": stream,
         "shard": shard,
+        "solve": solve,
         "batched": batched,
         "serving": serving,
         "large": large,
