@@ -410,17 +410,29 @@ struct PState {
     R rho, mx, my, mz, E, ux, uy, uz, p, s;   // s = |u| + c
 };
 
+// Wave speed |u| + c of a node state, computed once per staged node and kept
+// in its record (R arithmetic: FP64, or FP32 for mixed precision).
+template <typename R>
+__device__ __forceinline__ R part_speed(R rho, R mx, R my, R mz, R E)
+{
+    const R inv = part_rcp(rho);
+    const R ux = mx * inv, uy = my * inv, uz = mz * inv;
+    const R ssq = ux * ux + uy * uy + uz * uz;
+    const R p = R(kGamma - 1.0) * (E - R(0.5) * rho * ssq);
+    return part_sqrt(ssq) + part_sqrt(R(kGamma) * p * inv);
+}
+
+// An endpoint state from its record {rho, mx, my, mz, E, |u| + c}.
 __device__ __forceinline__ PState<double> part_state(const double* r)
 {
     const double2* q = reinterpret_cast<const double2*>(r);
     const double2 a = q[0], b = q[1], c = q[2];
     PState<double> t;
-    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
+    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x; t.s = c.y;
     const double inv = part_rcp(t.rho);
     t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
     const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
     t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
-    t.s = part_sqrt(ssq) + part_sqrt(kGamma * t.p * inv);
     return t;
 }
 
@@ -429,12 +441,11 @@ __device__ __forceinline__ PState<float> part_state32(const float* r)
     const float2* q = reinterpret_cast<const float2*>(r);
     const float2 a = q[0], b = q[1], c = q[2];
     PState<float> t;
-    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x;
+    t.rho = a.x; t.mx = a.y; t.my = b.x; t.mz = b.y; t.E = c.x; t.s = c.y;
     const float inv = part_rcp(t.rho);
     t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
     const float ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
     t.p = (float)(kGamma - 1.0) * (t.E - 0.5f * t.rho * ssq);
-    t.s = part_sqrt(ssq) + part_sqrt((float)kGamma * t.p * inv);
     return t;
 }
 
@@ -481,10 +492,10 @@ __device__ __forceinline__ void part_apply(double* acc, unsigned code, const O i
     }
 }
 
-// LDS node records of the part kernels: the raw state {rho, m, E} (6 values,
-// FP64 or FP32); both endpoint states are derived per edge -- the small record
-// lets more workgroups share a CU, which pays more than deriving each state
-// once.
+// LDS node records of the part kernels: the raw state {rho, m, E} and the
+// wave speed |u| + c (6 values, FP64 or FP32); the rest of each endpoint
+// state is derived per edge -- the small record lets more workgroups share a
+// CU, which pays more than deriving each state once.
 template <typename R>
 struct PartRec;
 template <>
@@ -498,14 +509,6 @@ struct PartRec<float> {
     static __device__ __forceinline__ PState<float> state(const float* r) { return part_state32(r); }
 };
 
-// Staging of one part, shared by the part kernels.  Staged state: flat
-// entry j = slot * 5 + v over the owned slots [0, cnt) and the halo slots
-// [cnt, cnt + nh), MAXJ entries per thread (MAXJ * T >= 5 * rec_cap).  The
-// halo index loads are unconditional (clamped into the part's halo list,
-// which the plan pads by one entry) so that they issue back to back under one
-// wait instead of one wait per divergent branch (part_halo_entries, then
-// part_stage_select).
-//
 // Marks a loaded register as needed here: the compiler waits for its load at
 // this point and cannot sink the load into a later divergent branch (where it
 // would be issued alone under its own wait).
@@ -515,30 +518,72 @@ __device__ __forceinline__ void part_ready(const V& v)
     asm volatile("" ::"v"(v));
 }
 
-// Raw halo-list entries of the staged slots (index loads only; the
-// select happens in part_stage_select once they are needed).
-template <int T, int MAXJ>
+// Staging of one part, shared by the part kernels: node slot sl = t + k T
+// (k < NPT, NPT * T >= rec_cap) of the owned nodes [0, cnt) and the halo
+// nodes [cnt, cnt + nh).  The halo-list loads are unconditional (clamped into
+// the part's list, which the plan pads by one entry) so that they issue back
+// to back under one wait instead of one wait per divergent branch; every
+// slot then names a valid node, so the state loads are unconditional too.
+template <int T, int NPT>
 __device__ __forceinline__ void part_halo_entries(const PartInfo& P, const int* __restrict__ halo, int t,
-                                                  int hn[MAXJ])
+                                                  int nd[NPT])
 {
     const int hmax = max(P.nh - 1, 0);
 #pragma unroll
-    for (int i = 0; i < MAXJ; i++) {
-        const int sl = (t + i * T) / 5;
-        hn[i] = halo[P.h0 + min(max(sl - P.cnt, 0), hmax)];
+    for (int k = 0; k < NPT; k++) nd[k] = halo[P.h0 + min(max(t + k * T - P.cnt, 0), hmax)];
+}
+
+template <int T, int NPT>
+__device__ __forceinline__ void part_load_nodes(const PartInfo& P, const double* __restrict__ var, int t,
+                                                int nd[NPT], double U[NPT][5])
+{
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        part_ready(nd[k]);
+        const int sl = t + k * T;
+        nd[k] = sl < P.cnt ? P.n0 + sl : nd[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NPT; k++)
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[k][v] = var[(size_t)nd[k] * 5 + v];
+}
+
+// Records {rho, m, E, |u| + c} of the staged slots.
+template <typename R, int T, int NPT, int RS>
+__device__ __forceinline__ void part_store_records(const PartInfo& P, int t, const double U[NPT][5], R* rec)
+{
+    const int nrec = P.cnt + P.nh;
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+        const int sl = t + k * T;
+        if (sl < nrec) {
+            const R x0 = (R)U[k][0], x1 = (R)U[k][1], x2 = (R)U[k][2], x3 = (R)U[k][3], x4 = (R)U[k][4];
+            R* r = rec + sl * RS;
+            r[0] = x0; r[1] = x1; r[2] = x2; r[3] = x3; r[4] = x4;
+            r[5] = part_speed<R>(x0, x1, x2, x3, x4);
+        }
     }
 }
 
-template <int T, int MAXJ>
-__device__ __forceinline__ void part_stage_select(const PartInfo& P, int t, const int hn[MAXJ], int src[MAXJ])
+// The owned nodes' old flux (FL flat entries per thread), or zeros for a
+// fresh start, into the FP64 accumulators.
+template <int T, int FL>
+__device__ __forceinline__ void part_load_flux(const PartInfo& P, const double* __restrict__ flux, int t,
+                                               bool fresh, double fl[FL])
+{
+    const double* fo = flux + (size_t)P.n0 * 5;
+#pragma unroll
+    for (int i = 0; i < FL; i++) fl[i] = fresh ? 0.0 : fo[min(t + i * T, P.cnt * 5 - 1)];
+}
+
+template <int T, int FL>
+__device__ __forceinline__ void part_store_flux(const PartInfo& P, int t, const double fl[FL], double* acc)
 {
 #pragma unroll
-    for (int i = 0; i < MAXJ; i++) part_ready(hn[i]);
-#pragma unroll
-    for (int i = 0; i < MAXJ; i++) {
+    for (int i = 0; i < FL; i++) {
         const int j = t + i * T;
-        const int sl = j / 5, v = j - sl * 5;
-        src[i] = (sl < P.cnt ? P.n0 + sl : hn[i]) * 5 + v;
+        if (j < P.cnt * 5) acc[j] = fl[i];
     }
 }
 
@@ -563,39 +608,6 @@ __device__ __forceinline__ void part_load_edges(const PartInfo& P, const unsigne
     }
 }
 
-// State entries and the owned nodes' old flux (FL entries per thread).
-template <int T, int MAXJ, int FL>
-__device__ __forceinline__ void part_load_state(const PartInfo& P, const double* __restrict__ var,
-                                                const double* __restrict__ flux, int t, const int src[MAXJ],
-                                                double st[MAXJ], double fl[FL], bool fresh)
-{
-    const double* fo = flux + (size_t)P.n0 * 5;
-#pragma unroll
-    for (int i = 0; i < MAXJ; i++) st[i] = var[src[i]];
-#pragma unroll
-    for (int i = 0; i < FL; i++) fl[i] = fresh ? 0.0 : fo[min(t + i * T, P.cnt * 5 - 1)];
-}
-
-template <typename R, int T, int MAXJ, int FL, int RS>
-__device__ __forceinline__ void part_store_lds(const PartInfo& P, int t, const double st[MAXJ],
-                                               const double fl[FL], R* rec, double* acc)
-{
-    const int nst = (P.cnt + P.nh) * 5;
-#pragma unroll
-    for (int i = 0; i < MAXJ; i++) {
-        const int j = t + i * T;
-        if (j < nst) {
-            const int sl = j / 5, v = j - sl * 5;
-            rec[sl * RS + v] = (R)st[i];
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < FL; i++) {
-        const int j = t + i * T;
-        if (j < P.cnt * 5) acc[j] = fl[i];
-    }
-}
-
 // The part's local edges, each increment added to its owned endpoints with
 // LDS atomics.
 template <typename R, int EPT, int RS>
@@ -617,8 +629,8 @@ __device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, con
 // accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
 // precision (records, weights and increment arithmetic in FP32, each
 // increment converted to FP64 and accumulated in FP64 from the FP64 flux).
-// One workgroup per part.
-template <typename R, int T, int EPT, int MAXJ, bool COLOURED>
+// One workgroup per part; NPT staged nodes per thread (NPT * T >= rec_cap).
+template <typename R, int T, int EPT, int NPT, bool COLOURED>
 __global__ __launch_bounds__(T) void k_flux_part(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
@@ -632,17 +644,18 @@ __global__ __launch_bounds__(T) void k_flux_part(
     const int t = threadIdx.x;
     R* rec = reinterpret_cast<R*>(sm);
     double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
-    // 1. loads: halo indices, edge codes and weights, then state and flux
-    int src[MAXJ];
+    // 1. loads: halo entries, edge codes and weights, then node states and
+    // the owned flux; records (with |u| + c) and accumulators into LDS
+    int nd[NPT];
     unsigned code[EPT];
     R w0[EPT], w1[EPT], w2[EPT];
-    double st[MAXJ], fl[FL];
-    int hn[MAXJ];
-    part_halo_entries<T, MAXJ>(P, halo, t, hn);
+    double U[NPT][5], fl[FL];
+    part_halo_entries<T, NPT>(P, halo, t, nd);
     part_load_edges<R, T, EPT>(P, ecode, ew, nloc, t, code, w0, w1, w2);
-    part_stage_select<T, MAXJ>(P, t, hn, src);
-    part_load_state<T, MAXJ, FL>(P, var, flux, t, src, st, fl, fresh != 0);
-    part_store_lds<R, T, MAXJ, FL, RS>(P, t, st, fl, rec, acc);
+    part_load_nodes<T, NPT>(P, var, t, nd, U);
+    part_load_flux<T, FL>(P, flux, t, fresh != 0, fl);
+    part_store_records<R, T, NPT, RS>(P, t, U, rec);
+    part_store_flux<T, FL>(P, t, fl, acc);
     __syncthreads();
     // 2./3. edges
     const R sc = (R)k.sc;

@@ -1171,30 +1171,30 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
     return check_launch();
 }
 
-template <typename Real, int T, int EPT, int MAXJ>
+template <typename Real, int T, int EPT, int NPT>
 static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                          const int* status, int fresh)
 {
     if (coloured)
-        k_flux_part<Real, T, EPT, MAXJ, true><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, true><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     else
-        k_flux_part<Real, T, EPT, MAXJ, false><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, false><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     return check_launch();
 }
 
-// Staged entries per thread: the smaller register array when the part plan's
+// Staged nodes per thread: the smaller register array when the part plan's
 // largest part allows it.
 template <typename Real, int T, int EPT>
 static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                          const int* status, int fresh)
 {
-    constexpr int kSmall = 3072 / T, kLarge = 4096 / T;
-    static_assert(kPartSlots * 5 <= kLarge * T, "staged entries exceed the kernel's registers");
-    if (P.pt_rec_cap * 5 <= kSmall * T)
+    constexpr int kSmall = (512 + T - 1) / T, kLarge = (768 + T - 1) / T;
+    static_assert(kPartSlots <= kLarge * T, "staged nodes exceed the kernel's registers");
+    if (P.pt_rec_cap <= kSmall * T)
         return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status, fresh);
     return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status, fresh);
 }
