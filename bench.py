@@ -97,6 +97,21 @@ def time_variant(rt, solvers, variant, iters, warmup):
     return ms.value
 
 
+def call_distribution(rt, solvers, variant, iters):
+    """Per-call times (an event between consecutive calls, rotating copies):
+    median, p10 and p90 in microseconds."""
+    n = len(solvers)
+    maps = (mgcfd.OpMap * n)(*[s.lv[0].edge_to_node for s in solvers])
+    vs = (mgcfd.OpDat * n)(*[s.lv[0].variables for s in solvers])
+    ws = (mgcfd.OpDat * n)(*[s.lv[0].edge_weights for s in solvers])
+    fs = (mgcfd.OpDat * n)(*[s.lv[0].fluxes for s in solvers])
+    each = (C.c_double * iters)()
+    mgcfd._check(rt.L.mgcfd_time_flux_calls(variant, n, maps, vs, ws, fs, iters, each), "mgcfd_time_flux_calls")
+    us = np.array(each[:]) * 1e3
+    return dict(calls=iters, median_us=round(float(np.median(us)), 2), p10_us=round(float(np.percentile(us, 10)), 2),
+                p90_us=round(float(np.percentile(us, 90)), 2))
+
+
 def time_stream(rt, solvers, iters, warmup):
     """The unstructured-stream loop (unstructured_stream.h:7-57): the flux
     loop's access pattern with almost no arithmetic, timed like the flux."""
@@ -476,6 +491,8 @@ def main():
     best = min(eligible, key=lambda k: eligible[k]["ms_rotating"])
     r = results[best]
     ms = r["ms_rotating"]
+    dist = call_distribution(rt, solvers, mgcfd.FLUX_VARIANTS[best], args.steps)
+    log(f"bench: {best} per-call distribution (rotating): {dist}")
     s_rot = time_stream(rt, solvers, args.steps, args.warmup)
     s_res = time_stream(rt, solvers[:1], args.steps, args.warmup)
     stream = dict(ms_rotating=round(s_rot, 6), ms_resident=round(s_res, 6),
@@ -527,6 +544,7 @@ def main():
                    "algorithmic_bytes_per_call": nb, "copy_kernel_GBs": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "per_call": dist,
         "cpu_baseline": cpu,
         "stream": stream,
         "shard": shard,

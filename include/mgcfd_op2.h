@@ -274,6 +274,12 @@ int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
 int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
                     const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
 
+/* The same launches with an event between consecutive calls: ms_each[iters]
+ * receives each call's time (for the distribution; the events between calls
+ * add a little to each). */
+int mgcfd_time_flux_calls(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
+                          const op_dat* weights, const op_dat* fluxes, int iters, double* ms_each);
+
 /* The same timing for the unstructured-stream loop (unstructured_stream.h:7-57,
  * which the reference runs beside the flux under -b, euler3d.cpp:518-523):
  * the flux loop's gathers and increments with almost no arithmetic, the

@@ -183,6 +183,34 @@ def test_time_stream_entry_matches_stream_loop(rt, orc, knn_small):
     assert bitwise(s.state(0, "dummy_fluxes"), o)
 
 
+def test_time_flux_calls_entry(rt, orc, knn_small):
+    """mgcfd_time_flux_calls (the bench's per-call distribution) runs the
+    flux loop once per call: two calls of the exact variant from zero fluxes
+    equal two sequential oracle passes bitwise, and each call has a positive
+    time."""
+    c = orc.freestream(0.5, 3.0)
+    s = mgcfd.Solver(rt, knn_small, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS["gather_exact"])
+    s.setup()
+    ref = oracle_setup(orc, knn_small, c)
+    m0 = knn_small[0]
+    n = len(m0["coords"])
+    h = s.lv[0]
+    var = perturbed_state(n, c.ff_variable[:], seed=35)
+    rt.upload(h.variables, var)
+    rt.upload(h.fluxes, np.zeros((n, 5)))
+    args = ((mgcfd.OpMap * 1)(h.edge_to_node), (mgcfd.OpDat * 1)(h.variables), (mgcfd.OpDat * 1)(h.edge_weights),
+            (mgcfd.OpDat * 1)(h.fluxes))
+    each = (C.c_double * 2)()
+    assert rt.L.mgcfd_time_flux_calls(mgcfd.FLUX_VARIANTS["gather_exact"], 1, *args, 2, each) == 0
+    o = np.zeros_like(var)
+    e2n = np.ascontiguousarray(m0["edge_to_node"], np.int32)
+    orc.compute_flux_edge(e2n, var, ref[0]["ew"], o, c)
+    orc.compute_flux_edge(e2n, var, ref[0]["ew"], o, c)
+    assert bitwise(s.state(0, "fluxes"), o)
+    assert each[0] > 0 and each[1] > 0
+
+
 def test_time_stepping_loops(rt, orc, knn_small):
     """copy, calculate_dt, get_min_dt, step factor, time_step, residual, rms, bad values."""
     c = orc.freestream(0.5, 3.0)
