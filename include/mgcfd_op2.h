@@ -181,6 +181,19 @@ int op_par_loop_up_post_kernel(const char* name, op_set set, op_arg a0, op_arg a
 /* mg.h:66-88, site euler3d.cpp:626-631 */
 int op_par_loop_down_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2,
                             op_arg a3, op_arg a4);
+/* Inverse-distance prolongation, mg.h:90-204, sites euler3d.cpp:606-623.
+ * The reference driver runs these only when its edge-->mg_nodes map is
+ * declared, which it never is (euler3d.cpp:286-292); mgcfd_decl_map_compose
+ * builds that map.  down_v2_kernel takes the ten arguments of
+ * euler3d.cpp:609-619 (fine coords and prolonged sums through edge-->node,
+ * coarse coords and residuals through edge-->mg_nodes) and is bitwise OP2's
+ * sequential backend, including its OP_INC-by-pointer overwrite for fine
+ * nodes that coincide with their coarse parent (mg.h:124-132,160-167). */
+int op_par_loop_down_v2_kernel_pre(const char* name, op_set set, op_arg a0, op_arg a1);
+int op_par_loop_down_v2_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2, op_arg a3,
+                               op_arg a4, op_arg a5, op_arg a6, op_arg a7, op_arg a8, op_arg a9);
+int op_par_loop_down_v2_kernel_post(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2,
+                                    op_arg a3);
 /* validation.h:46-89, site euler3d.cpp:690-693 */
 int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
 /* validation.h:91-100, site euler3d.cpp:696-698 (gbl OP_INC) */
@@ -192,6 +205,13 @@ int op_par_loop_count_non_zeros(const char* name, op_set set, op_arg a0, op_arg 
 int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* args);
 
 /* ---- mgcfd_* extensions (no OP2 counterpart) ---- */
+
+/* The composition C[e][k] = B[A[e][k]] of a map A: X -> Y and a dimension-1
+ * map B: Y -> Z, as a new map X -> Z of A's dimension: the edge-->mg_nodes
+ * map that euler3d.cpp:286-292 describes (edge-->node, then node-->mg_node)
+ * and leaves undeclared.  Its columns may name one node twice.  Returns
+ * NULL on error (mgcfd_last_error). */
+op_map mgcfd_decl_map_compose(op_map outer, op_map inner, const char* name);
 
 /* Edge-flux implementation used by op_par_loop_compute_flux_edge_kernel:
  *   MGCFD_FLUX_GATHER_EXACT  blocked gather: one workgroup per mesh part

@@ -48,6 +48,9 @@ OP_KERNEL(up_pre_kernel);
 OP_KERNEL(up_kernel);
 OP_KERNEL(up_post_kernel);
 OP_KERNEL(down_kernel);
+OP_KERNEL(down_v2_kernel_pre);
+OP_KERNEL(down_v2_kernel);
+OP_KERNEL(down_v2_kernel_post);
 OP_KERNEL(identify_differences);
 OP_KERNEL(count_non_zeros);
 
@@ -62,7 +65,7 @@ struct Config {
     int cycles = 25, flow_interval = 0;
     bool out_variables = false, out_fluxes = false, out_step_factors = false;
     double mach = 1.2, aoa = 0.0;
-    bool device_loop = false, print_rms = false;
+    bool device_loop = false, print_rms = false, down_v2 = false;
     std::string flux_variant = "gather_exact";
 };
 
@@ -127,12 +130,14 @@ void print_help()
                  "-v, --validate                 check final state against solution files\n"
                  "-I, --output-flow-interval=INT write L0 variables every INT cycles\n"
                  "--output-variables --output-fluxes --output-step-factors\n"
-                 "--mach M --aoa DEG --device-loop --flux-variant NAME --validate-strict --print-rms\n");
+                 "--mach M --aoa DEG --device-loop --flux-variant NAME --validate-strict --print-rms\n"
+                 "--down-v2                      prolong by inverse distance (mg.h:90-204) over the\n"
+                 "                               edge-->mg_nodes map built from edge-->node and node-->mg_node\n");
 }
 
 bool parse_args(Config& c, int argc, char** argv)
 {
-    enum { OPT_OUTV = 1000, OPT_OUTF, OPT_OUTS, OPT_MACH, OPT_AOA, OPT_DEV, OPT_FV, OPT_STRICT, OPT_RMS };
+    enum { OPT_OUTV = 1000, OPT_OUTF, OPT_OUTS, OPT_MACH, OPT_AOA, OPT_DEV, OPT_FV, OPT_STRICT, OPT_RMS, OPT_DV2 };
     static struct option opts[] = {
         {"help", no_argument, nullptr, 'h'},
         {"config-filepath", required_argument, nullptr, 'c'},
@@ -157,6 +162,7 @@ bool parse_args(Config& c, int argc, char** argv)
         {"flux-variant", required_argument, nullptr, OPT_FV},
         {"validate-strict", no_argument, nullptr, OPT_STRICT},
         {"print-rms", no_argument, nullptr, OPT_RMS},
+        {"down-v2", no_argument, nullptr, OPT_DV2},
         {nullptr, 0, nullptr, 0},
     };
     int o;
@@ -183,6 +189,7 @@ bool parse_args(Config& c, int argc, char** argv)
             case OPT_FV: c.flux_variant = optarg; break;
             case OPT_STRICT: c.validate = c.strict = true; break;
             case OPT_RMS: c.print_rms = true; break;
+            case OPT_DV2: c.down_v2 = true; break;
             default: std::printf("Unknown command line parameter '%c'\n", o);
         }
     }
@@ -314,6 +321,10 @@ int main(int argc, char** argv)
         op_printf("ERROR: input_file not set\n");
         return 1;
     }
+    if (conf.down_v2 && conf.device_loop) {
+        op_printf("ERROR: --down-v2 runs loop by loop; the device-resident loop prolongs with down_kernel\n");
+        return 1;
+    }
     const int variant = flux_variant_id(conf.flux_variant);
     if (variant < 0) {
         std::fprintf(stderr, "ERROR: unknown --flux-variant '%s'\n", conf.flux_variant.c_str());
@@ -349,10 +360,10 @@ int main(int argc, char** argv)
     op_decl_const(1, "int", &mesh_name);
 
     std::vector<op_set> nodes(levels), edges(levels), bnd(levels);
-    std::vector<op_map> e2n(levels), b2n(levels), n2mg(std::max(levels - 1, 0));
+    std::vector<op_map> e2n(levels), b2n(levels), n2mg(std::max(levels - 1, 0)), e2mg(levels, nullptr);
     std::vector<op_dat> ew(levels), bw(levels), bg(levels), xyz(levels), correct(levels, nullptr);
     std::vector<op_dat> var(levels), old(levels), res(levels), vol(levels, nullptr), sf(levels), flux(levels),
-        dummy(levels, nullptr), scratch(levels, nullptr);
+        dummy(levels, nullptr), scratch(levels, nullptr), rpro(levels, nullptr), rwsum(levels, nullptr);
     const std::string rn = conf.legacy ? ".renumbered" : "";
     char name[128];
 
@@ -370,9 +381,15 @@ int main(int argc, char** argv)
         e2n[i] = op_decl_map_hdf5(edges[i], nodes[i], 2, L, ("edge-->node" + rn).c_str());
         b2n[i] = op_decl_map_hdf5(bnd[i], nodes[i], 1, L, ("bnd_node-->node" + rn).c_str());
         bg[i] = op_decl_dat_hdf5(bnd[i], 1, "int", L, "bnd_node-->group");
-        if (i > 0)
+        if (i > 0) {
             n2mg[i - 1] = op_decl_map_hdf5(nodes[i - 1], nodes[i], 1, layers[i - 1].c_str(),
                                            ("node-->mg_node" + rn).c_str());
+            // the map euler3d.cpp:286-292 describes and leaves undeclared
+            if (conf.down_v2) {
+                e2mg[i - 1] = mgcfd_decl_map_compose(e2n[i - 1], n2mg[i - 1], "edge-->mg_nodes");
+                if (!e2mg[i - 1]) op_check_(-1, "mgcfd_decl_map_compose");
+            }
+        }
         if (conf.legacy) {
             vol[i] = op_decl_dat_hdf5(nodes[i], 1, "double", L, "areas");
             std::snprintf(name, sizeof name, "p_volumes_L%d", i);
@@ -414,6 +431,10 @@ int main(int argc, char** argv)
         flux[i] = temp(NVAR, "double", "p_fluxes_L%d");
         if (conf.mem_bound) dummy[i] = temp(NVAR, "double", "p_dummy_fluxes_L%d");
         if (i > 0) scratch[i] = temp(1, "int", "p_up_scratch_L%d");   // euler3d.cpp:405
+        if (e2mg[i]) {
+            rpro[i] = temp(NVAR, "double", "p_residuals_prolonged_L%d");
+            rwsum[i] = temp(1, "double", "p_residuals_prolonged_wsum_L%d");
+        }
     }
 
     // euler3d.cpp:412-441
@@ -582,12 +603,35 @@ int main(int argc, char** argv)
                 if (level == levels - 1) up = false;
             } else {
                 level--;
-                op_par_loop(down_kernel, "down_kernel", nodes[level],
-                            op_arg_dat(var[level], -1, OP_ID, NVAR, "double", OP_INC),
-                            op_arg_dat(res[level], -1, OP_ID, NVAR, "double", OP_READ),
-                            op_arg_dat(xyz[level], -1, OP_ID, NDIM, "double", OP_READ),
-                            op_arg_dat(res[level + 1], 0, n2mg[level], NVAR, "double", OP_READ),
-                            op_arg_dat(xyz[level + 1], 0, n2mg[level], NDIM, "double", OP_READ));
+                if (e2mg[level]) {      // euler3d.cpp:603-623
+                    op_map en = e2n[level], em = e2mg[level];
+                    op_par_loop(down_v2_kernel_pre, "down_v2_kernel_pre", nodes[level],
+                                op_arg_dat(rwsum[level], -1, OP_ID, 1, "double", OP_WRITE),
+                                op_arg_dat(rpro[level], -1, OP_ID, NVAR, "double", OP_WRITE));
+                    op_par_loop(down_v2_kernel, "down_v2_kernel", edges[level],
+                                op_arg_dat(xyz[level], 0, en, NDIM, "double", OP_READ),
+                                op_arg_dat(xyz[level], 1, en, NDIM, "double", OP_READ),
+                                op_arg_dat(xyz[level + 1], 0, em, NDIM, "double", OP_READ),
+                                op_arg_dat(xyz[level + 1], 1, em, NDIM, "double", OP_READ),
+                                op_arg_dat(res[level + 1], 0, em, NVAR, "double", OP_READ),
+                                op_arg_dat(res[level + 1], 1, em, NVAR, "double", OP_READ),
+                                op_arg_dat(rpro[level], 0, en, NVAR, "double", OP_INC),
+                                op_arg_dat(rpro[level], 1, en, NVAR, "double", OP_INC),
+                                op_arg_dat(rwsum[level], 0, en, 1, "double", OP_INC),
+                                op_arg_dat(rwsum[level], 1, en, 1, "double", OP_INC));
+                    op_par_loop(down_v2_kernel_post, "down_v2_kernel_post", nodes[level],
+                                op_arg_dat(rpro[level], -1, OP_ID, NVAR, "double", OP_READ),
+                                op_arg_dat(rwsum[level], -1, OP_ID, 1, "double", OP_READ),
+                                op_arg_dat(res[level], -1, OP_ID, NVAR, "double", OP_READ),
+                                op_arg_dat(var[level], -1, OP_ID, NVAR, "double", OP_INC));
+                } else {                // euler3d.cpp:625-631
+                    op_par_loop(down_kernel, "down_kernel", nodes[level],
+                                op_arg_dat(var[level], -1, OP_ID, NVAR, "double", OP_INC),
+                                op_arg_dat(res[level], -1, OP_ID, NVAR, "double", OP_READ),
+                                op_arg_dat(xyz[level], -1, OP_ID, NDIM, "double", OP_READ),
+                                op_arg_dat(res[level + 1], 0, n2mg[level], NVAR, "double", OP_READ),
+                                op_arg_dat(xyz[level + 1], 0, n2mg[level], NDIM, "double", OP_READ));
+                }
                 if (level == 0) { up = true; i++; }
             }
         }

@@ -1522,6 +1522,83 @@ __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__
     store5(vf, i, u);
 }
 
+// Inverse distance 1 / |p - q| as mg.h:134,147,170,183 form it (exact:
+// no contraction, IEEE square root and division).
+__device__ __forceinline__ double down_v2_idist(const double* __restrict__ p, const double* __restrict__ q)
+{
+#pragma clang fp contract(off)
+    const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+    return 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// down_v2_kernel (mg.h:101-191) as a gather: one thread per fine node walks
+// its incidences of the edge map in element order (the OP2 sequential order)
+// and applies, for its own side of each edge, what the reference applies
+// through its OP_INC pointers:
+//   - the node coincides with its coarse parent: the sums are set to the
+//     parent's residual and 1 (mg.h:124-132, 160-167), on every such edge;
+//   - side a: + w(a, parent a) r(parent a) + w(parent b, a) r(parent b);
+//   - side b: + w(b, parent b) r(parent b) + w(parent a, b) r(parent b) --
+//     the reference's b side takes b's parent residual in both terms.
+// The sums start from the dats' values (OP_INC), so the result is bitwise
+// the sequential one.  e2mg is the edge-->mg_nodes map, internal ids.
+__global__ __launch_bounds__(kBlock) void k_down_v2(
+    int n_targets, const int* __restrict__ targets, const int* __restrict__ ptr, const int2* __restrict__ inc,
+    const int* __restrict__ e2mg, const double* __restrict__ xf, const double* __restrict__ xc,
+    const double* __restrict__ rc, double* __restrict__ rp, double* __restrict__ wsum)
+{
+#pragma clang fp contract(off)
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n_targets) return;
+    const int n = targets ? targets[tid] : tid;
+    const double x2[3] = {xf[3 * (size_t)n], xf[3 * (size_t)n + 1], xf[3 * (size_t)n + 2]};
+    double acc[5], w = wsum[n];
+    load5(rp, n, acc);
+    for (int q = ptr[tid]; q < ptr[tid + 1]; q++) {
+        const unsigned code = (unsigned)inc[q].y;
+        const size_t e = code & 0x7fffffffu;
+        const int side = (int)(code >> 31);
+        const int own = e2mg[2 * e + side], oth = e2mg[2 * e + (1 - side)];
+        const double* x1 = xc + 3 * (size_t)own;
+        double r1[5];
+        load5(rc, own, r1);
+        if (x2[0] - x1[0] == 0.0 && x2[1] - x1[1] == 0.0 && x2[2] - x1[2] == 0.0) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) acc[v] = r1[v];
+            w = 1.0;
+            continue;
+        }
+        double d = down_v2_idist(x2, x1);
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v] += d * r1[v];
+        w += d;
+        d = down_v2_idist(xc + 3 * (size_t)oth, x2);
+        if (side == 0) load5(rc, oth, r1);
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v] += d * r1[v];
+        w += d;
+    }
+    store5(rp, n, acc);
+    wsum[n] = w;
+}
+
+// down_v2_kernel_post (mg.h:193-204): var += res - rp / wsum.
+__global__ void k_down_v2_post(int n, const double* __restrict__ rp, const double* __restrict__ wsum,
+                               const double* __restrict__ rf, double* __restrict__ vf)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[5], r[5], p[5];
+    load5(vf, i, u);
+    load5(rf, i, r);
+    load5(rp, i, p);
+    const double w = wsum[i];
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] += r[v] - (p[v] / w);
+    store5(vf, i, u);
+}
+
 // down_kernel (mg.h:66-88) followed by the fine level's calculate_dt.
 __global__ __launch_bounds__(kBlock) void k_down_dt(
     int nf, const int* __restrict__ f2c, double* __restrict__ vf, const double* __restrict__ rf,

@@ -121,6 +121,7 @@ struct InvPlan {
 
 struct MapImpl {
     bool on_device = false;
+    bool composed = false;       // mgcfd_decl_map_compose: may repeat a target, plays no part in ordering
     std::vector<int> h_orig;     // [from][dim], original ids, 0-based
     int* d_map = nullptr;        // [from internal][dim], internal ids
     InvPlan inv;
@@ -459,7 +460,7 @@ static int freeze()
         if (!s.impl || S(&s)->frozen) continue;
         NodeSet x{&s, {}, {}};
         for (auto& m : R.maps)
-            if (m.impl && m.to == &s && m.dim == 2 && m.from != &s) {
+            if (m.impl && m.to == &s && m.dim == 2 && m.from != &s && !M(&m)->composed) {
                 x.ems.push_back(M(&m));
                 x.fsz.push_back(m.from->size);
             }
@@ -490,7 +491,8 @@ static int freeze()
         if (si->frozen || !si->perm.empty() || s.size == 0) continue;
         op_map key = nullptr;
         for (auto& m : R.maps)
-            if (m.impl && m.from == &s && m.to->impl && !S(m.to)->perm.empty() && (key == nullptr || m.dim > key->dim))
+            if (m.impl && !M(&m)->composed && m.from == &s && m.to->impl && !S(m.to)->perm.empty() &&
+                (key == nullptr || m.dim > key->dim))
                 key = &m;
         if (key) es.emplace_back(&s, key);
     }
@@ -1363,6 +1365,8 @@ op_set op_decl_set(int size, const char* name)
     return s;
 }
 
+static op_map add_map(op_set from, op_set to, int dim, MapImpl* mi, const char* name);
+
 op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char* name)
 {
     if (!from || !to || dim < 1 || (!imap && from->size > 0)) { fail("op_decl_map %s: bad arguments", name ? name : ""); return nullptr; }
@@ -1386,6 +1390,11 @@ op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char*
                      e, mi->h_orig[2 * (size_t)e]);
                 return nullptr;
             }
+    return add_map(from, to, dim, mi, name);
+}
+
+static op_map add_map(op_set from, op_set to, int dim, MapImpl* mi, const char* name)
+{
     R.maps.push_back(op_map_core{});
     op_map m = &R.maps.back();
     m->index = (int)R.maps.size() - 1;
@@ -1396,6 +1405,23 @@ op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char*
     m->impl = mi;
     R.dirty = true;
     return m;
+}
+
+op_map mgcfd_decl_map_compose(op_map outer, op_map inner, const char* name)
+{
+    if (!R.inited) { fail("mgcfd_decl_map_compose: op_init has not been called"); return nullptr; }
+    if (!outer || !inner || !outer->impl || !inner->impl || inner->dim != 1 || outer->to != inner->from) {
+        fail("mgcfd_decl_map_compose %s: needs maps A: X -> Y and B: Y -> Z with B of dimension 1",
+             name ? name : "");
+        return nullptr;
+    }
+    const MapImpl* a = M(outer);
+    const MapImpl* b = M(inner);
+    MapImpl* mi = new MapImpl();
+    mi->composed = true;
+    mi->h_orig.resize(a->h_orig.size());
+    for (size_t q = 0; q < a->h_orig.size(); q++) mi->h_orig[q] = b->h_orig[(size_t)a->h_orig[q]];
+    return add_map(outer->from, inner->to, outer->dim, mi, name);
 }
 
 static op_dat decl_dat(op_set set, int dim, const char* type, int size, const char* data, const char* name)
@@ -1976,6 +2002,75 @@ int op_par_loop_down_kernel(const char* name, op_set set, op_arg a0, op_arg a1, 
     return check_launch();
 }
 
+int op_par_loop_down_v2_kernel_pre(const char* name, op_set set, op_arg a0, op_arg a1)
+{
+    const char* L = "down_v2_kernel_pre";
+    (void)name;
+    TRY(chk_ready(L));
+    TRY(direct(a0, set, 1, false, L, "arg0 prolonged weight sums OP_WRITE"));
+    TRY(direct(a1, set, 5, false, L, "arg1 prolonged residuals[5] OP_WRITE"));
+    const size_t n = (size_t)set->size;
+    if (n) {
+        k_fill<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(a0.dat), 0.0);
+        k_fill<<<nblk(5 * n), kBlock, 0, R.stream>>>(5 * n, dptr(a1.dat), 0.0);
+    }
+    touch(a0.dat);
+    touch(a1.dat);
+    return check_launch();
+}
+
+int op_par_loop_down_v2_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2, op_arg a3,
+                               op_arg a4, op_arg a5, op_arg a6, op_arg a7, op_arg a8, op_arg a9)
+{
+    const char* L = "down_v2_kernel";
+    (void)name;
+    TRY(chk_ready(L));
+    TRY(indirect(a0, set, 0, 2, 3, L, "arg0 fine coords via edge-->node[0]"));
+    TRY(indirect(a1, set, 1, 2, 3, L, "arg1 fine coords via edge-->node[1]"));
+    TRY(indirect(a2, set, 0, 2, 3, L, "arg2 coarse coords via edge-->mg_nodes[0]"));
+    TRY(indirect(a3, set, 1, 2, 3, L, "arg3 coarse coords via edge-->mg_nodes[1]"));
+    TRY(indirect(a4, set, 0, 2, 5, L, "arg4 coarse residuals via edge-->mg_nodes[0]"));
+    TRY(indirect(a5, set, 1, 2, 5, L, "arg5 coarse residuals via edge-->mg_nodes[1]"));
+    TRY(indirect(a6, set, 0, 2, 5, L, "arg6 prolonged residuals via edge-->node[0] OP_INC"));
+    TRY(indirect(a7, set, 1, 2, 5, L, "arg7 prolonged residuals via edge-->node[1] OP_INC"));
+    TRY(indirect(a8, set, 0, 2, 1, L, "arg8 prolonged weight sums via edge-->node[0] OP_INC"));
+    TRY(indirect(a9, set, 1, 2, 1, L, "arg9 prolonged weight sums via edge-->node[1] OP_INC"));
+    const op_map en = a0.map, em = a2.map;
+    TRY(need(a1.map == en && a6.map == en && a7.map == en && a8.map == en && a9.map == en, L,
+             "fine arguments through one edge-->node map"));
+    TRY(need(a3.map == em && a4.map == em && a5.map == em, L, "coarse arguments through one edge-->mg_nodes map"));
+    TRY(need(a0.dat == a1.dat && a2.dat == a3.dat && a4.dat == a5.dat && a6.dat == a7.dat && a8.dat == a9.dat,
+             L, "one dat per argument pair"));
+    TRY(need(a6.acc == OP_INC && a7.acc == OP_INC && a8.acc == OP_INC && a9.acc == OP_INC, L,
+             "prolonged sums OP_INC"));
+    TRY(need(a6.dat != a4.dat && a6.dat != a0.dat && a6.dat != a2.dat, L, "prolonged residuals distinct from inputs"));
+    TRY(build_inverse(en));
+    const InvPlan& P = M(en)->inv;
+    if (P.n_targets)
+        k_down_v2<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2,
+                                                               M(em)->d_map, dptr(a0.dat), dptr(a2.dat),
+                                                               dptr(a4.dat), dptr(a6.dat), dptr(a8.dat));
+    touch(a6.dat);
+    touch(a8.dat);
+    return check_launch();
+}
+
+int op_par_loop_down_v2_kernel_post(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2, op_arg a3)
+{
+    const char* L = "down_v2_kernel_post";
+    (void)name;
+    TRY(chk_ready(L));
+    TRY(direct(a0, set, 5, false, L, "arg0 prolonged residuals[5] OP_READ"));
+    TRY(direct(a1, set, 1, false, L, "arg1 prolonged weight sums OP_READ"));
+    TRY(direct(a2, set, 5, false, L, "arg2 residuals[5] OP_READ"));
+    TRY(direct(a3, set, 5, false, L, "arg3 variables[5] OP_INC"));
+    if (set->size)
+        k_down_v2_post<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat),
+                                                                 dptr(a2.dat), dptr(a3.dat));
+    touch(a3.dat);
+    return check_launch();
+}
+
 int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2)
 {
     const char* L = "identify_differences";
@@ -2032,6 +2127,9 @@ int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a
         {"up_kernel", 3, (void*)op_par_loop_up_kernel},
         {"up_post_kernel", 2, (void*)op_par_loop_up_post_kernel},
         {"down_kernel", 5, (void*)op_par_loop_down_kernel},
+        {"down_v2_kernel_pre", 2, (void*)op_par_loop_down_v2_kernel_pre},
+        {"down_v2_kernel", 10, (void*)op_par_loop_down_v2_kernel},
+        {"down_v2_kernel_post", 4, (void*)op_par_loop_down_v2_kernel_post},
         {"identify_differences", 3, (void*)op_par_loop_identify_differences},
         {"count_non_zeros", 2, (void*)op_par_loop_count_non_zeros},
     };
@@ -2043,12 +2141,15 @@ int op_par_loop_by_name(const char* name, op_set set, int nargs, const op_arg* a
         using F3 = int (*)(const char*, op_set, op_arg, op_arg, op_arg);
         using F4 = int (*)(const char*, op_set, op_arg, op_arg, op_arg, op_arg);
         using F5 = int (*)(const char*, op_set, op_arg, op_arg, op_arg, op_arg, op_arg);
+        using F10 = int (*)(const char*, op_set, op_arg, op_arg, op_arg, op_arg, op_arg, op_arg, op_arg, op_arg,
+                            op_arg, op_arg);
         switch (nargs) {
             case 1: return ((F1)e.f)(name, set, a[0]);
             case 2: return ((F2)e.f)(name, set, a[0], a[1]);
             case 3: return ((F3)e.f)(name, set, a[0], a[1], a[2]);
             case 4: return ((F4)e.f)(name, set, a[0], a[1], a[2], a[3]);
-            default: return ((F5)e.f)(name, set, a[0], a[1], a[2], a[3], a[4]);
+            case 5: return ((F5)e.f)(name, set, a[0], a[1], a[2], a[3], a[4]);
+            default: return ((F10)e.f)(name, set, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9]);
         }
     }
     return fail("op_par_loop: no device implementation of loop '%s'", name);

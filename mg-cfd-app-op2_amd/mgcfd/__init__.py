@@ -93,6 +93,8 @@ def lib() -> C.CDLL:
     L.op_decl_set.restype = OpSet
     L.op_decl_set.argtypes = [C.c_int, C.c_char_p]
     L.op_decl_map.restype = OpMap
+    L.mgcfd_decl_map_compose.restype = OpMap
+    L.mgcfd_decl_map_compose.argtypes = [OpMap, OpMap, C.c_char_p]
     L.op_decl_map.argtypes = [OpSet, OpSet, C.c_int, C.c_void_p, C.c_char_p]
     L.op_decl_dat_char.restype = OpDat
     L.op_decl_dat_char.argtypes = [OpSet, C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_char_p]
@@ -130,7 +132,8 @@ LOOPS = {
     "get_min_dt_kernel": 2, "compute_step_factor_kernel": 4, "compute_flux_edge_kernel": 5,
     "compute_bnd_node_flux_kernel": 4, "time_step_kernel": 5, "unstructured_stream_kernel": 5,
     "residual_kernel": 3, "calc_rms_kernel": 2, "count_bad_vals": 2, "up_pre_kernel": 2, "up_kernel": 3,
-    "up_post_kernel": 2, "down_kernel": 5, "identify_differences": 3, "count_non_zeros": 2,
+    "up_post_kernel": 2, "down_kernel": 5, "down_v2_kernel_pre": 2, "down_v2_kernel": 10,
+    "down_v2_kernel_post": 4, "identify_differences": 3, "count_non_zeros": 2,
 }
 
 
@@ -277,6 +280,13 @@ class Runtime:
     def decl_map(self, frm, to, dim: int, data: np.ndarray, name: str):
         a = np.ascontiguousarray(data, dtype=np.int32)
         m = self.L.op_decl_map(frm, to, dim, a.ctypes.data, name.encode())
+        if not m:
+            raise MgcfdError(last_error())
+        return m
+
+    def compose_map(self, outer, inner, name: str):
+        """C[e][k] = inner[outer[e][k]] (mgcfd_decl_map_compose)."""
+        m = self.L.mgcfd_decl_map_compose(outer, inner, name.encode())
         if not m:
             raise MgcfdError(last_error())
         return m
@@ -452,6 +462,9 @@ class LevelHandles:
     edge_to_node: object = None
     bnd_to_node: object = None
     node_to_mg_node: object = None
+    edge_to_mg_nodes: object = None
+    prolonged: object = None
+    prolonged_wsum: object = None
     coords: object = None
     edge_weights: object = None
     bnd_weights: object = None
@@ -471,11 +484,14 @@ class Solver:
     """Declares a multigrid mesh as euler3d.cpp:243-409 does and runs its loops.
 
     ``mesh`` is a list of per-level dicts (see mesh_box / mesh_knn).
+    ``down_v2`` declares the edge-->mg_nodes maps (euler3d.cpp:286-292) and
+    prolongs with the inverse-distance loops (euler3d.cpp:603-623).
     """
 
     def __init__(self, rt: Runtime, mesh: list[dict], mach: float = 1.2, aoa_deg: float = 0.0,
-                 measure_mem_bound: bool = False):
+                 measure_mem_bound: bool = False, down_v2: bool = False):
         self.rt = rt
+        self.down_v2 = down_v2
         self.nlev = len(mesh)
         self.fs = freestream(mach, aoa_deg)
         rt.decl_consts(self.fs)
@@ -500,6 +516,11 @@ class Solver:
         for i in range(1, self.nlev):
             self.lv[i - 1].node_to_mg_node = rt.decl_map(self.lv[i - 1].nodes, self.lv[i].nodes, 1,
                                                          mesh[i - 1]["node_to_mg_node"], f"op_node-->mg_node_L{i}")
+            if down_v2:
+                f = self.lv[i - 1]
+                f.edge_to_mg_nodes = rt.compose_map(f.edge_to_node, f.node_to_mg_node, f"op_edge-->mg_nodes_L{i}")
+                f.prolonged = rt.decl_dat(f.nodes, 5, None, f"p_residuals_prolonged_L{i - 1}")
+                f.prolonged_wsum = rt.decl_dat(f.nodes, 1, None, f"p_residuals_prolonged_wsum_L{i - 1}")
         for i, h in enumerate(self.lv):
             h.variables = rt.decl_dat(h.nodes, 5, None, f"p_variables_L{i}")
             h.old_variables = rt.decl_dat(h.nodes, 5, None, f"p_old_variables_L{i}")
@@ -611,8 +632,22 @@ class Solver:
                     rt.arg(c.up_scratch, -1, None, 1, "int", OP_READ))
 
     def prolong(self, l: int):
-        """MG_DOWN to level l (euler3d.cpp:601-631, down_kernel branch)."""
+        """MG_DOWN to level l (euler3d.cpp:601-631): the inverse-distance loops
+        when the edge-->mg_nodes map is declared, else down_kernel."""
         rt, f, c = self.rt, self.lv[l], self.lv[l + 1]
+        if f.edge_to_mg_nodes:
+            en, em, R, I = f.edge_to_node, f.edge_to_mg_nodes, OP_READ, OP_INC
+            rt.par_loop("down_v2_kernel_pre", f.nodes, rt.arg(f.prolonged_wsum, -1, None, 1, acc=OP_WRITE),
+                        rt.arg(f.prolonged, -1, None, 5, acc=OP_WRITE))
+            rt.par_loop("down_v2_kernel", f.edges, rt.arg(f.coords, 0, en, 3, acc=R), rt.arg(f.coords, 1, en, 3, acc=R),
+                        rt.arg(c.coords, 0, em, 3, acc=R), rt.arg(c.coords, 1, em, 3, acc=R),
+                        rt.arg(c.residuals, 0, em, 5, acc=R), rt.arg(c.residuals, 1, em, 5, acc=R),
+                        rt.arg(f.prolonged, 0, en, 5, acc=I), rt.arg(f.prolonged, 1, en, 5, acc=I),
+                        rt.arg(f.prolonged_wsum, 0, en, 1, acc=I), rt.arg(f.prolonged_wsum, 1, en, 1, acc=I))
+            rt.par_loop("down_v2_kernel_post", f.nodes, rt.arg(f.prolonged, -1, None, 5, acc=R),
+                        rt.arg(f.prolonged_wsum, -1, None, 1, acc=R), rt.arg(f.residuals, -1, None, 5, acc=R),
+                        rt.arg(f.variables, -1, None, 5, acc=I))
+            return
         m = f.node_to_mg_node
         rt.par_loop("down_kernel", f.nodes, rt.arg(f.variables, -1, None, 5, acc=OP_INC),
                     rt.arg(f.residuals, -1, None, 5, acc=OP_READ), rt.arg(f.coords, -1, None, 3, acc=OP_READ),
@@ -647,6 +682,9 @@ class Solver:
         return 0, rms_hist, mdt
 
     def bind(self):
+        if self.down_v2:
+            raise MgcfdError("the device-resident multigrid prolongs with down_kernel; "
+                             "a down_v2 solver runs loop by loop (run_loops)")
         binds = (LevelBind * self.nlev)()
         for i, h in enumerate(self.lv):
             b = binds[i]

@@ -460,6 +460,75 @@ void orc_down(int n_fine, const int* f2c, double* var_f, const double* res_f,
     }
 }
 
+/* Inverse-distance prolongation (mg.h:90-204, sites euler3d.cpp:603-624).
+ * The reference driver never declares the edge-->mg_nodes map it needs
+ * (euler3d.cpp:286-292 leaves it NULL), so these loops only run where a
+ * caller supplies it: e2mg[e][k] = node_to_mg_node[e2n[e][k]].
+ * down_v2_kernel_pre :90-100 over fine nodes */
+void orc_down_v2_pre(int n, double* wsum, double* rp)
+{
+    for (size_t i = 0; i < (size_t)n; i++) {
+        wsum[i] = 0.0;
+        for (int v = 0; v < 5; v++) rp[5 * i + v] = 0.0;
+    }
+}
+
+static inline double inv_dist(const double* p, const double* q)
+{
+    double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+    return 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* down_v2_kernel :101-191 over edges in element order.  Its OP_INC
+ * arguments are pointers into the dats (OP2 sequential backend), so a fine
+ * node that coincides with its coarse parent has its sums overwritten
+ * (:124-132, :160-167) -- by the same values on every edge.  Side a adds
+ * its parent's residual and the residual of b's parent (:133-152); side b
+ * adds its own parent's residual twice, the second time weighted by the
+ * distance to a's parent (:178-189: residuals1b where a's would be
+ * expected). */
+void orc_down_v2(int n_edges, const int* e2n, const int* e2mg, const double* xyz_f,
+                 const double* xyz_c, const double* res_c, double* rp, double* wsum)
+{
+    for (size_t e = 0; e < (size_t)n_edges; e++) {
+        size_t a = (size_t)e2n[2 * e], b = (size_t)e2n[2 * e + 1];
+        size_t ca = (size_t)e2mg[2 * e], cb = (size_t)e2mg[2 * e + 1];
+        const double *x2a = xyz_f + 3 * a, *x2b = xyz_f + 3 * b;
+        const double *x1a = xyz_c + 3 * ca, *x1b = xyz_c + 3 * cb;
+        const double *r1a = res_c + 5 * ca, *r1b = res_c + 5 * cb;
+        double *pa = rp + 5 * a, *pb = rp + 5 * b;
+        if (x2a[0] - x1a[0] == 0.0 && x2a[1] - x1a[1] == 0.0 && x2a[2] - x1a[2] == 0.0) {
+            for (int v = 0; v < 5; v++) pa[v] = r1a[v];
+            wsum[a] = 1.0;
+        } else {
+            double d = inv_dist(x2a, x1a);
+            for (int v = 0; v < 5; v++) pa[v] += d * r1a[v];
+            wsum[a] += d;
+            d = inv_dist(x1b, x2a);
+            for (int v = 0; v < 5; v++) pa[v] += d * r1b[v];
+            wsum[a] += d;
+        }
+        if (x2b[0] - x1b[0] == 0.0 && x2b[1] - x1b[1] == 0.0 && x2b[2] - x1b[2] == 0.0) {
+            for (int v = 0; v < 5; v++) pb[v] = r1b[v];
+            wsum[b] = 1.0;
+        } else {
+            double d = inv_dist(x2b, x1b);
+            for (int v = 0; v < 5; v++) pb[v] += d * r1b[v];
+            wsum[b] += d;
+            d = inv_dist(x1a, x2b);
+            for (int v = 0; v < 5; v++) pb[v] += d * r1b[v];
+            wsum[b] += d;
+        }
+    }
+}
+
+/* down_v2_kernel_post :193-204 over fine nodes */
+void orc_down_v2_post(int n, const double* rp, const double* wsum, const double* res_f, double* var_f)
+{
+    for (size_t i = 0; i < (size_t)n; i++)
+        for (int v = 0; v < 5; v++) var_f[5 * i + v] += res_f[5 * i + v] - (rp[5 * i + v] / wsum[i]);
+}
+
 /* ---------------- driver: euler3d.cpp:412-641 --------------------------- */
 
 void orc_setup(orc_level* L, int levels, const orc_consts* c)
@@ -529,8 +598,16 @@ int orc_mg_run(orc_level* L, int levels, int cycles, const orc_consts* c,
             level--;
             orc_level* f = &L[level];
             orc_level* cl = &L[level + 1];
-            orc_down(f->n_nodes, f->node_to_mg_node, f->variables, f->residuals, f->coords,
-                     cl->residuals, cl->coords);
+            if (f->edge_to_mg_nodes) {                                    /* :603-623 */
+                orc_down_v2_pre(f->n_nodes, f->prolonged_wsum, f->prolonged);
+                orc_down_v2(f->n_edges, f->edge_to_node, f->edge_to_mg_nodes, f->coords, cl->coords,
+                            cl->residuals, f->prolonged, f->prolonged_wsum);
+                orc_down_v2_post(f->n_nodes, f->prolonged, f->prolonged_wsum, f->residuals,
+                                 f->variables);
+            } else {                                                      /* :625-631 */
+                orc_down(f->n_nodes, f->node_to_mg_node, f->variables, f->residuals, f->coords,
+                         cl->residuals, cl->coords);
+            }
             if (level == 0) { up = 1; i++; }
         }
     }

@@ -60,6 +60,11 @@ typedef struct {
     double* step_factors;         /* [N]    */
     double* volumes;              /* [N]    */
     int*    up_scratch;           /* [N]    */
+    /* Inverse-distance prolongation (mg.h:90-204) onto this level when
+     * edge_to_mg_nodes is set ([E][2] -> level+1); NULL: down_kernel. */
+    const int* edge_to_mg_nodes;  /* [E][2] */
+    double* prolonged;            /* [N][5] */
+    double* prolonged_wsum;       /* [N]    */
 } orc_level;
 
 /* ---- per-loop restatements (OP2 seq order) ---- */
@@ -91,6 +96,10 @@ void orc_up(int n_fine, const int* f2c, const double* var_f, double* var_c, int*
 void orc_up_post(int n_coarse, double* var_c, const int* scratch_c);
 void orc_down(int n_fine, const int* f2c, double* var_f, const double* res_f,
               const double* xyz_f, const double* res_c, const double* xyz_c);
+void orc_down_v2_pre(int n, double* wsum, double* rp);
+void orc_down_v2(int n_edges, const int* e2n, const int* e2mg, const double* xyz_f,
+                 const double* xyz_c, const double* res_c, double* rp, double* wsum);
+void orc_down_v2_post(int n, const double* rp, const double* wsum, const double* res_f, double* var_f);
 void orc_identify_differences(int n, const double* test, const double* master, double* diff);
 int orc_count_non_zeros(int n, const double* diff);
 

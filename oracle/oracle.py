@@ -42,7 +42,8 @@ class Level(C.Structure):
                 ("edge_to_node", _PI), ("bnd_to_node", _PI), ("bnd_group", _PI), ("node_to_mg_node", _PI),
                 ("coords", _PD), ("edge_weights", _PD), ("bnd_weights", _PD), ("variables", _PD),
                 ("old_variables", _PD), ("residuals", _PD), ("fluxes", _PD), ("step_factors", _PD),
-                ("volumes", _PD), ("up_scratch", _PI)]
+                ("volumes", _PD), ("up_scratch", _PI),
+                ("edge_to_mg_nodes", _PI), ("prolonged", _PD), ("prolonged_wsum", _PD)]
 
 
 def _p(a: np.ndarray):
@@ -138,6 +139,15 @@ class Oracle:
     def down(self, f2c, var_f, res_f, xyz_f, res_c, xyz_c):
         self.fn("down")(len(f2c), _p(f2c), _p(var_f), _p(res_f), _p(xyz_f), _p(res_c), _p(xyz_c))
 
+    def down_v2_pre(self, wsum, rp):
+        self.fn("down_v2_pre")(len(wsum), _p(wsum), _p(rp))
+
+    def down_v2(self, e2n, e2mg, xyz_f, xyz_c, res_c, rp, wsum):
+        self.fn("down_v2")(len(e2n), _p(e2n), _p(e2mg), _p(xyz_f), _p(xyz_c), _p(res_c), _p(rp), _p(wsum))
+
+    def down_v2_post(self, rp, wsum, res_f, var_f):
+        self.fn("down_v2_post")(len(wsum), _p(rp), _p(wsum), _p(res_f), _p(var_f))
+
     def identify_differences(self, test, master, diff):
         self.fn("identify_differences")(len(test), _p(test), _p(master), _p(diff))
 
@@ -160,10 +170,12 @@ class Oracle:
 
     # --- whole solve
     def solve(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, consts: Consts | None = None,
-              after_setup=None):
+              after_setup=None, down_v2=False):
         """Setup (euler3d.cpp:412-441) + multigrid loop (:452-641) on copies of
         the mesh arrays.  after_setup(list of per-level array dicts) may modify
-        the state between the two (failure-path tests).  Returns
+        the state between the two (failure-path tests).  down_v2 prolongs with
+        the inverse-distance loops (euler3d.cpp:603-623) over the edge-->mg_nodes
+        map e2mg[e][k] = node_to_mg_node[e2n[e][k]].  Returns
         dict(status, rms, min_dt, steps, levels=[...])."""
         c = consts if consts is not None else self.freestream(mach, aoa)
         nl = len(mesh)
@@ -181,7 +193,10 @@ class Oracle:
                 ew=np.array(m["edge_weights"], dtype=np.float64, copy=True),
                 bw=np.array(m["bnd_weights"], dtype=np.float64, copy=True),
                 var=np.zeros((n, 5)), old=np.zeros((n, 5)), res=np.zeros((n, 5)), flux=np.zeros((n, 5)),
-                sf=np.zeros(n), vol=np.zeros(n), scr=np.zeros(n, dtype=np.int32))
+                sf=np.zeros(n), vol=np.zeros(n), scr=np.zeros(n, dtype=np.int32), e2mg=None,
+                rp=np.zeros((n, 5)), wsum=np.zeros(n))
+            if down_v2 and a["n2c"] is not None:
+                a["e2mg"] = np.ascontiguousarray(a["n2c"][a["e2n"]], dtype=np.int32)
             arrs.append(a)
             L = levels[i]
             L.n_nodes, L.n_edges, L.n_bnd = n, e, b
@@ -190,6 +205,8 @@ class Oracle:
             L.coords, L.edge_weights, L.bnd_weights = _p(a["xyz"]), _p(a["ew"]), _p(a["bw"])
             L.variables, L.old_variables, L.residuals = _p(a["var"]), _p(a["old"]), _p(a["res"])
             L.fluxes, L.step_factors, L.volumes, L.up_scratch = _p(a["flux"]), _p(a["sf"]), _p(a["vol"]), _p(a["scr"])
+            L.edge_to_mg_nodes = _p(a["e2mg"]) if a["e2mg"] is not None else None
+            L.prolonged, L.prolonged_wsum = _p(a["rp"]), _p(a["wsum"])
         self.fn("setup")(levels, nl, C.byref(c))
         if after_setup is not None:
             after_setup(arrs)
@@ -201,5 +218,5 @@ class Oracle:
         for a in arrs:
             out.append(dict(variables=a["var"], old_variables=a["old"], residuals=a["res"], fluxes=a["flux"],
                             step_factors=a["sf"], volumes=a["vol"], edge_weights=a["ew"], bnd_weights=a["bw"],
-                            up_scratch=a["scr"]))
+                            up_scratch=a["scr"], prolonged=a["rp"], prolonged_wsum=a["wsum"]))
         return dict(status=st, rms=rms[:cycles], min_dt=mdt[:steps.value], steps=steps.value, levels=out)

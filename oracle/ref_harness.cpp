@@ -269,6 +269,28 @@ void ref_down(int n_fine, const int* f2c, double* var_f, const double* res_f, co
     }
 }
 
+void ref_down_v2_pre(int n, double* wsum, double* rp)
+{
+    for (size_t i = 0; i < (size_t)n; i++) down_v2_kernel_pre(wsum + i, rp + 5 * i);
+}
+
+// OP2 sequential semantics: the OP_INC arguments point into the dats.
+void ref_down_v2(int n_edges, const int* e2n, const int* e2mg, const double* xyz_f, const double* xyz_c,
+                 const double* res_c, double* rp, double* wsum)
+{
+    for (size_t e = 0; e < (size_t)n_edges; e++) {
+        size_t a = (size_t)e2n[2 * e], b = (size_t)e2n[2 * e + 1];
+        size_t ca = (size_t)e2mg[2 * e], cb = (size_t)e2mg[2 * e + 1];
+        down_v2_kernel(xyz_f + 3 * a, xyz_f + 3 * b, xyz_c + 3 * ca, xyz_c + 3 * cb, res_c + 5 * ca,
+                       res_c + 5 * cb, rp + 5 * a, rp + 5 * b, wsum + a, wsum + b);
+    }
+}
+
+void ref_down_v2_post(int n, const double* rp, const double* wsum, const double* res_f, double* var_f)
+{
+    for (size_t i = 0; i < (size_t)n; i++) down_v2_kernel_post(rp + 5 * i, wsum + i, res_f + 5 * i, var_f + 5 * i);
+}
+
 void ref_identify_differences(int n, const double* test, const double* master, double* diff)
 {
     for (size_t i = 0; i < (size_t)n; i++)
@@ -343,8 +365,15 @@ int ref_mg_run(orc_level* L, int nlev, int cycles, const orc_consts* c, double* 
             level--;
             orc_level* f = &L[level];
             orc_level* cl = &L[level + 1];
-            ref_down(f->n_nodes, f->node_to_mg_node, f->variables, f->residuals, f->coords,
-                     cl->residuals, cl->coords);
+            if (f->edge_to_mg_nodes) {
+                ref_down_v2_pre(f->n_nodes, f->prolonged_wsum, f->prolonged);
+                ref_down_v2(f->n_edges, f->edge_to_node, f->edge_to_mg_nodes, f->coords, cl->coords,
+                            cl->residuals, f->prolonged, f->prolonged_wsum);
+                ref_down_v2_post(f->n_nodes, f->prolonged, f->prolonged_wsum, f->residuals, f->variables);
+            } else {
+                ref_down(f->n_nodes, f->node_to_mg_node, f->variables, f->residuals, f->coords,
+                         cl->residuals, cl->coords);
+            }
             if (level == 0) { mg_dir = MG_UP; i++; }
         }
     }

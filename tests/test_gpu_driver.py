@@ -123,3 +123,23 @@ def test_driver_flow_interval_snapshots(deck, tmp_path):
     last = h5_read(L, f"{prefix}variables.L0.cycle={cycles}.h5", "p_variables")
     o = ref["levels"][0]["variables"]
     assert (np.abs(last - o).max(axis=0) / np.abs(o).max(axis=0)).max() <= 1e-12
+
+
+def test_driver_down_v2_matches_oracle(deck, tmp_path):
+    """--down-v2: the driver declares edge-->mg_nodes (euler3d.cpp:286-292)
+    and prolongs with the inverse-distance loops (euler3d.cpp:603-623); the
+    final state of every level matches the oracle's run of the same loops."""
+    d, cycles, _, L = deck
+    ref = Oracle("orc").solve(read_deck(L, d, 3), cycles, 0.5, 3.0, down_v2=True)
+    assert ref["status"] == 0
+    prefix = str(tmp_path) + "/v2."
+    rc, out = run(d, "-g", str(cycles), "--mach", "0.5", "--aoa", "3", "--down-v2", "--output-variables",
+                  "-o", prefix)
+    assert rc == 0, out
+    for l in range(3):
+        g = h5_read(L, f"{prefix}variables.L{l}.cycles={cycles}.h5", f"p_variables_result_L{l}")
+        o = ref["levels"][l]["variables"]
+        err = (np.abs(g - o).max(axis=0) / np.abs(o).max(axis=0)).max()
+        assert err <= 1e-12, (l, err)
+    rc, out = run(d, "-g", "1", "--down-v2", "--device-loop")
+    assert rc != 0 and "--down-v2 runs loop by loop" in out, out

@@ -313,6 +313,80 @@ def test_multigrid_transfer_bitwise(rt, orc, meshname, request):
     assert bitwise(s.state(0, "variables"), of)
 
 
+@pytest.mark.parametrize("meshname", MESHES)
+def test_down_v2_loops_bitwise(rt, orc, meshname, request):
+    """Inverse-distance prolongation (down_v2_kernel_pre / _kernel / _post)
+    over the composed edge-->mg_nodes map, bitwise against the oracle.  The
+    kNN mesh has fine nodes that coincide with their coarse parent (the
+    overwrite branch) and edges whose ends share a parent."""
+    mesh = request.getfixturevalue(meshname)
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0, down_v2=True)
+    s.setup()
+    nf, nc = len(mesh[0]["coords"]), len(mesh[1]["coords"])
+    e2n = np.ascontiguousarray(mesh[0]["edge_to_node"], np.int32)
+    n2c = np.asarray(mesh[0]["node_to_mg_node"], np.int32)
+    e2mg = np.ascontiguousarray(n2c[e2n])
+    xf, xc = np.ascontiguousarray(mesh[0]["coords"]), np.ascontiguousarray(mesh[1]["coords"])
+    c = orc.freestream(0.5, 3.0)
+    vf = perturbed_state(nf, c.ff_variable[:], seed=71)
+    rf = perturbed_state(nf, np.ones(5), seed=72) * 1e-5
+    rc = perturbed_state(nc, np.ones(5), seed=73) * 1e-5
+    rt.upload(s.lv[0].variables, vf)
+    rt.upload(s.lv[0].residuals, rf)
+    rt.upload(s.lv[1].residuals, rc)
+    rt.upload(s.lv[0].prolonged, np.full((nf, 5), 7.0))   # the pre loop must clear these
+    s.prolong(0)
+    rp, ws, of = np.zeros((nf, 5)), np.zeros(nf), vf.copy()
+    orc.down_v2_pre(ws, rp)
+    orc.down_v2(e2n, e2mg, xf, xc, rc, rp, ws)
+    orc.down_v2_post(rp, ws, rf, of)
+    assert bitwise(rt.fetch(s.lv[0].prolonged), rp)
+    assert bitwise(rt.fetch(s.lv[0].prolonged_wsum)[:, 0], ws)
+    assert bitwise(s.state(0, "variables"), of)
+
+
+def test_down_v2_hand_mesh(rt):
+    """Three fine nodes, two coarse nodes, every fine-to-coarse distance 5.
+    n1 coincides with its parent: its sums are set, not added, on each of its
+    two edges (wsum 1, not 2).  n0, side b of its second edge, takes its own
+    parent's residual in both b-side terms (mg.h:178-189)."""
+    fn, fe, cn = rt.decl_set(3, "op_nodes_L0"), rt.decl_set(3, "op_edges_L0"), rt.decl_set(2, "op_nodes_L1")
+    e2n = rt.decl_map(fe, fn, 2, np.array([[0, 1], [2, 0], [1, 2]]), "edge-->node")
+    n2c = rt.decl_map(fn, cn, 1, np.array([0, 1, 1]), "node-->mg_node")
+    e2mg = rt.compose_map(e2n, n2c, "edge-->mg_nodes")
+    xf = rt.decl_dat(fn, 3, np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 5.0], [3.0, 4.0, 5.0]]), "coords_L0")
+    xc = rt.decl_dat(cn, 3, np.array([[3.0, 4.0, 0.0], [0.0, 0.0, 5.0]]), "coords_L1")
+    r0, r1 = np.array([1.0, 2.0, -3.0, 0.5, 4.0]), np.array([-2.0, 1.0, 1.0, 8.0, -1.0])
+    rc = rt.decl_dat(cn, 5, np.stack([r0, r1]), "residuals_L1")
+    rf = rt.decl_dat(fn, 5, np.array([[0.5] * 5, [0.25] * 5, [-1.0] * 5]), "residuals_L0")
+    var = rt.decl_dat(fn, 5, np.full((3, 5), 2.0), "variables_L0")
+    rp, ws = rt.decl_dat(fn, 5, None, "prolonged"), rt.decl_dat(fn, 1, None, "prolonged_wsum")
+    R, W, I = mgcfd.OP_READ, mgcfd.OP_WRITE, mgcfd.OP_INC
+    rt.par_loop("down_v2_kernel_pre", fn, rt.arg(ws, -1, None, 1, acc=W), rt.arg(rp, -1, None, 5, acc=W))
+    rt.par_loop("down_v2_kernel", fe, rt.arg(xf, 0, e2n, 3, acc=R), rt.arg(xf, 1, e2n, 3, acc=R),
+                rt.arg(xc, 0, e2mg, 3, acc=R), rt.arg(xc, 1, e2mg, 3, acc=R),
+                rt.arg(rc, 0, e2mg, 5, acc=R), rt.arg(rc, 1, e2mg, 5, acc=R),
+                rt.arg(rp, 0, e2n, 5, acc=I), rt.arg(rp, 1, e2n, 5, acc=I),
+                rt.arg(ws, 0, e2n, 1, acc=I), rt.arg(ws, 1, e2n, 1, acc=I))
+    p, w = rt.fetch(rp), rt.fetch(ws)[:, 0]
+    assert np.allclose(p[0], 0.6 * r0 + 0.2 * r1, rtol=1e-15, atol=1e-16) and np.isclose(w[0], 0.8, rtol=1e-15)
+    assert np.all(p[1] == r1) and w[1] == 1.0
+    assert np.allclose(p[2], 0.2 * r0 + 0.6 * r1, rtol=1e-15, atol=1e-16) and np.isclose(w[2], 0.8, rtol=1e-15)
+    rt.par_loop("down_v2_kernel_post", fn, rt.arg(rp, -1, None, 5, acc=R), rt.arg(ws, -1, None, 1, acc=R),
+                rt.arg(rf, -1, None, 5, acc=R), rt.arg(var, -1, None, 5, acc=I))
+    v = rt.fetch(var)
+    assert np.allclose(v[0], 2.5 - (0.75 * r0 + 0.25 * r1), rtol=1e-15)
+    assert np.all(v[1] == 2.25 - r1)
+    assert np.allclose(v[2], 1.0 - (0.25 * r0 + 0.75 * r1), rtol=1e-15)
+    # the weight sums passed OP_READ instead of OP_INC: refused, not run
+    with pytest.raises(mgcfd.MgcfdError, match="argument mismatch"):
+        rt.par_loop("down_v2_kernel", fe, rt.arg(xf, 0, e2n, 3, acc=R), rt.arg(xf, 1, e2n, 3, acc=R),
+                    rt.arg(xc, 0, e2mg, 3, acc=R), rt.arg(xc, 1, e2mg, 3, acc=R),
+                    rt.arg(rc, 0, e2mg, 5, acc=R), rt.arg(rc, 1, e2mg, 5, acc=R),
+                    rt.arg(rp, 0, e2n, 5, acc=I), rt.arg(rp, 1, e2n, 5, acc=I),
+                    rt.arg(ws, 0, e2n, 1, acc=R), rt.arg(ws, 1, e2n, 1, acc=R))
+
+
 def test_identify_differences(rt, orc, box8):
     s = mgcfd.Solver(rt, box8, 0.5, 3.0)
     s.setup()
@@ -361,6 +435,21 @@ def test_solve_parity(rt, orc, meshname, mode, request):
     for l in range(len(mesh)):
         assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
     assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
+
+
+def test_solve_parity_down_v2(rt, orc, knn_small):
+    """A 3-cycle solve prolonging with the inverse-distance loops, against
+    the oracle's run of the same loop order (euler3d.cpp:603-623)."""
+    ref = orc.solve(knn_small, 3, 0.5, 3.0, down_v2=True)
+    s = mgcfd.Solver(rt, knn_small, 0.5, 3.0, down_v2=True)
+    s.setup()
+    st, rms, _ = s.run_loops(3)
+    assert st == 0 and ref["status"] == 0
+    for l in range(len(knn_small)):
+        assert rel_err(s.state(l), ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
+    assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
+    with pytest.raises(mgcfd.MgcfdError, match="loop by loop"):
+        s.run_device(1)
 
 
 @pytest.mark.parametrize("variant", ["gather_exact", "part_atomic", "mixed_atomic"])

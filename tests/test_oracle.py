@@ -11,6 +11,7 @@
    freestream speed of sound, a<->b swap symmetry, zero weights, uniform-state
    dissipation, conservation, boundary branches, the closed forms of dt, the
    step factor and the RK coefficients, restriction as a mean, prolongation,
+   the inverse-distance prolongation on a hand mesh and as a weighted mean,
    cell volumes of a regular lattice, and the -v tolerance rule.
 """
 import os
@@ -165,6 +166,51 @@ def test_oracle_equals_reference_solve(orc, knn_small):
     assert bits_equal(a["min_dt"], b["min_dt"]) and bits_equal(a["rms"], b["rms"])
     for la, lb in zip(a["levels"], b["levels"]):
         assert bits_equal(la["variables"], lb["variables"])
+
+
+@needs_ref
+@pytest.mark.parametrize("level", [0, 1])
+def test_oracle_equals_reference_down_v2(orc, knn_small, level):
+    """Inverse-distance prolongation loops (mg.h:90-204) in OP2 sequential
+    order, on a kNN level whose fine nodes partly coincide with their coarse
+    parents (both branches) and whose edges partly share one parent."""
+    ref = Oracle("ref")
+    f, c = knn_small[level], knn_small[level + 1]
+    e2n = np.ascontiguousarray(f["edge_to_node"], np.int32)
+    n2c = np.asarray(f["node_to_mg_node"], np.int32)
+    e2mg = np.ascontiguousarray(n2c[e2n], np.int32)
+    xf, xc = np.asarray(f["coords"], np.float64), np.asarray(c["coords"], np.float64)
+    coinc = np.all(xf == xc[n2c], axis=1)
+    assert 0 < coinc.sum() < len(xf)
+    rng = np.random.default_rng(11 + level)
+    rc = rng.uniform(-1e-3, 1e-3, (len(xc), 5))
+    rf = rng.uniform(-1e-3, 1e-3, (len(xf), 5))
+    outs = []
+    for o in (orc, ref):
+        rp, ws = np.full((len(xf), 5), 3.0), np.full(len(xf), 3.0)
+        o.down_v2_pre(ws, rp)
+        assert np.all(rp == 0.0) and np.all(ws == 0.0)
+        o.down_v2(e2n, e2mg, xf, xc, rc, rp, ws)
+        var = np.ones((len(xf), 5))
+        o.down_v2_post(rp, ws, rf, var)
+        outs.append((rp, ws, var))
+    for x, y in zip(*outs):
+        assert bits_equal(x, y)
+    rp, ws, _ = outs[0]
+    assert np.all(ws[coinc] == 1.0) and bits_equal(rp[coinc], rc[n2c[coinc]])
+
+
+@needs_ref
+def test_oracle_equals_reference_solve_down_v2(orc, knn_small):
+    ref = Oracle("ref")
+    a = orc.solve(knn_small, 3, 0.6, 1.0, down_v2=True)
+    b = ref.solve(knn_small, 3, 0.6, 1.0, down_v2=True)
+    assert a["status"] == b["status"] == 0
+    assert bits_equal(a["min_dt"], b["min_dt"]) and bits_equal(a["rms"], b["rms"])
+    for la, lb in zip(a["levels"], b["levels"]):
+        assert bits_equal(la["variables"], lb["variables"])
+    plain = orc.solve(knn_small, 3, 0.6, 1.0)
+    assert not bits_equal(plain["levels"][0]["variables"], a["levels"][0]["variables"])
 
 
 # ------------------------------------------------------------ closed forms
@@ -340,6 +386,48 @@ def test_prolong_formula(orc):
     expect = np.array([1 - dm * dr[0], 1 - d[0] * dr[1], 1 - d[1] * dr[2], 1 - d[2] * dr[3], 1 - dm * dr[4]])
     assert np.allclose(vf[0], expect, rtol=1e-15, atol=1e-15)
     assert np.all(vf[1] == 1.0)            # coincident nodes: no correction
+
+
+def test_down_v2_hand_mesh(orc):
+    """Three fine nodes, two coarse nodes, every fine-to-coarse distance 5
+    (weight 1/5).  n1 coincides with its parent C1, so its sums are set (not
+    added) on each of its two edges.  n0 is side b of e1, whose b-side terms
+    both take b's parent residual r0 (mg.h:178-189), so n0 gets 3 r0 + r1
+    rather than 2 r0 + 2 r1."""
+    xc = np.array([[3.0, 4.0, 0.0], [0.0, 0.0, 5.0]])             # C0, C1
+    xf = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 5.0], [3.0, 4.0, 5.0]])
+    n2c = np.array([0, 1, 1], np.int32)
+    e2n = np.array([[0, 1], [2, 0], [1, 2]], np.int32)
+    e2mg = np.ascontiguousarray(n2c[e2n])
+    rc = np.array([[1.0, 2.0, -3.0, 0.5, 4.0], [-2.0, 1.0, 1.0, 8.0, -1.0]])
+    r0, r1 = rc
+    rp, ws = np.zeros((3, 5)), np.zeros(3)
+    orc.down_v2(e2n, e2mg, xf, xc, rc, rp, ws)
+    assert np.allclose(rp[0], 0.6 * r0 + 0.2 * r1, rtol=1e-15, atol=1e-16) and np.isclose(ws[0], 0.8, rtol=1e-15)
+    assert np.all(rp[1] == r1) and ws[1] == 1.0
+    assert np.allclose(rp[2], 0.2 * r0 + 0.6 * r1, rtol=1e-15, atol=1e-16) and np.isclose(ws[2], 0.8, rtol=1e-15)
+    rf = np.array([[0.5] * 5, [0.25] * 5, [-1.0] * 5])
+    var = np.full((3, 5), 2.0)
+    orc.down_v2_post(rp, ws, rf, var)
+    assert np.allclose(var[0], 2.0 + 0.5 - (0.75 * r0 + 0.25 * r1), rtol=1e-15)
+    assert np.all(var[1] == 2.0 + 0.25 - r1)
+    assert np.allclose(var[2], 2.0 - 1.0 - (0.25 * r0 + 0.75 * r1), rtol=1e-15)
+
+
+def test_down_v2_uniform_coarse_residual(orc, knn_small):
+    """A weighted mean of equal values: with one residual on every coarse
+    node the prolonged residual rp / wsum is that residual at every fine node
+    with an edge."""
+    f, c = knn_small[0], knn_small[1]
+    e2n = np.ascontiguousarray(f["edge_to_node"], np.int32)
+    n2c = np.asarray(f["node_to_mg_node"], np.int32)
+    xf, xc = np.asarray(f["coords"], np.float64), np.asarray(c["coords"], np.float64)
+    r = np.array([1e-3, -2e-3, 5e-4, 0.0, 3e-3])
+    rc = np.tile(r, (len(xc), 1))
+    rp, ws = np.zeros((len(xf), 5)), np.zeros(len(xf))
+    orc.down_v2(e2n, np.ascontiguousarray(n2c[e2n]), xf, xc, rc, rp, ws)
+    assert np.all(ws > 0)
+    assert np.allclose(rp / ws[:, None], r[None, :], rtol=1e-13, atol=1e-18)
 
 
 def test_cell_volumes_regular_lattice(orc):
