@@ -97,6 +97,14 @@ def kernel_vectors(ref: Oracle, mesh: list[dict], seed: int) -> dict:
     vf = var.copy()
     ref.down(f2c, vf, res, out["coords"], rc, out["coords_c"])
     out["res_c"], out["down_var"] = rc, vf
+    # inverse-distance prolongation over edge-->mg_nodes = node-->mg_node o edge-->node
+    e2mg = np.ascontiguousarray(f2c[e2n], np.int32)
+    rp, ws = np.full((n, 5), 9.0), np.full(n, 9.0)
+    ref.down_v2_pre(ws, rp)
+    ref.down_v2(e2n, e2mg, out["coords"], out["coords_c"], rc, rp, ws)
+    vf2 = var.copy()
+    ref.down_v2_post(rp, ws, res, vf2)
+    out["dv2_rp"], out["dv2_wsum"], out["dv2_var"] = rp, ws, vf2
     # -v comparator on a perturbed copy
     test = var.copy()
     test[::7, 2] *= 1 + 3e-7

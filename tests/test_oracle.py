@@ -99,6 +99,12 @@ def test_golden_kernels(orc, fixture):
     vf = g["var"].copy()
     orc.down(g["f2c"], vf, g["residual"], g["coords"], g["res_c"], g["coords_c"])
     assert bits_equal(vf, g["down_var"])
+    rp, ws = np.zeros((n, 5)), np.zeros(n)
+    orc.down_v2(g["e2n"], np.ascontiguousarray(g["f2c"][g["e2n"]]), g["coords"], g["coords_c"], g["res_c"], rp, ws)
+    assert bits_equal(rp, g["dv2_rp"]) and bits_equal(ws, g["dv2_wsum"])
+    vf = g["var"].copy()
+    orc.down_v2_post(rp, ws, g["residual"], vf)
+    assert bits_equal(vf, g["dv2_var"])
     diff = np.zeros((n, 5))
     orc.identify_differences(g["v_test"], g["var"], diff)
     assert bits_equal(diff, g["v_diff"])
