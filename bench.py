@@ -469,8 +469,13 @@ def main():
 
     st = (C.c_int * 5)()
     mgcfd._check(rt.L.mgcfd_plan_stats(solvers[0].lv[0].edge_to_node, st), "mgcfd_plan_stats")
-    plan = dict(workgroups=st[0], staged_slots=st[1], incidences=st[2], max_degree=st[3], lds_bytes=st[4])
-    log(f"bench: edge-block plan {plan}")
+    gather_plan = dict(workgroups=st[0], staged_slots=st[1], incidences=st[2], max_degree=st[3], lds_bytes=st[4])
+    log(f"bench: edge-block plan {gather_plan}")
+    st = (C.c_int * 6)()
+    mgcfd._check(rt.L.mgcfd_part_plan_stats(solvers[0].lv[0].edge_to_node, st), "mgcfd_part_plan_stats")
+    plan = dict(parts=st[0], max_staged_nodes=st[1], median_staged_nodes=st[2], local_edges=st[3],
+                max_local_edges=st[4], lds_bytes=st[5])
+    log(f"bench: part plan {plan}")
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -539,7 +544,7 @@ def main():
         "data": "synthetic: seeded jittered-point kNN mesh, freestream M0.5 a3deg +-1% perturbation",
         "config": {"workload": "compute_flux_edge_kernel, level 0 of the 300K-node/930K-edge kNN mesh",
                    "n_nodes": N_NODES, "n_edges": N_EDGES, "variant": best,
-                   "mesh_copies_rotated": args.copies, "plan": plan,
+                   "mesh_copies_rotated": args.copies, "part_plan": plan, "gather_plan": gather_plan,
                    "working_set_MB": round(args.copies * nb / 1e6, 1),
                    "algorithmic_bytes_per_call": nb, "copy_kernel_GBs": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

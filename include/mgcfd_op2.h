@@ -255,6 +255,13 @@ int mgcfd_get_flux_variant(void);
  * edge), [3] max node degree, [4] dynamic LDS bytes per workgroup. */
 int mgcfd_plan_stats(op_map map, int* stats);
 
+/* Part plan of an edge->node map (variants PART_* / MIXED_*), built if
+ * needed: stats[0] parts (one workgroup each), [1] largest and [2] median
+ * staged node records of a part (owned + halo), [3] local edges over all
+ * parts (cut edges count once per side), [4] largest local edges of a part,
+ * [5] dynamic LDS bytes per workgroup of the FP64 kernel. */
+int mgcfd_part_plan_stats(op_map map, int* stats);
+
 /* Last error message of this thread ("" if none). */
 const char* mgcfd_last_error(void);
 

@@ -99,6 +99,7 @@ struct InvPlan {
     int pt_parts = 0;            // 0 = not available
     long long pt_nloc = 0;       // local edges over all parts
     int pt_rec_cap = 0;          // max staged nodes (owned + halo) of one part
+    int pt_rec_med = 0;          // median staged nodes of a part
     int pt_max_ne = 0;           // max local edges of one part
     int pt_max_col = 0;
     size_t pt_lds = 0;
@@ -878,6 +879,12 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     P.pt_parts = (int)parts.size();
     P.pt_nloc = (long long)eid.size();
     P.pt_rec_cap = rec_cap;
+    {
+        std::vector<int> r(parts.size());
+        for (size_t i = 0; i < parts.size(); i++) r[i] = parts[i].cnt + parts[i].nh;
+        std::nth_element(r.begin(), r.begin() + r.size() / 2, r.end());
+        P.pt_rec_med = r.empty() ? 0 : r[r.size() / 2];
+    }
     P.pt_max_ne = max_ne;
     P.pt_max_col = max_col;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
@@ -1589,6 +1596,21 @@ int mgcfd_plan_stats(op_map map, int* stats)
     stats[2] = P.bk_entries;
     stats[3] = P.max_deg;
     stats[4] = (int)P.bk_lds;
+    return 0;
+}
+
+int mgcfd_part_plan_stats(op_map map, int* stats)
+{
+    if (!map || !stats) return fail("mgcfd_part_plan_stats: null argument");
+    TRY(ensure_flux_plan(map, MGCFD_FLUX_PART_ATOMIC));
+    const InvPlan& P = M(map)->inv;
+    if (P.pt_parts == 0) return fail("mgcfd_part_plan_stats: map %s has no part plan", map->name);
+    stats[0] = P.pt_parts;
+    stats[1] = P.pt_rec_cap;
+    stats[2] = P.pt_rec_med;
+    stats[3] = (int)P.pt_nloc;
+    stats[4] = P.pt_max_ne;
+    stats[5] = (int)P.pt_lds;
     return 0;
 }
 

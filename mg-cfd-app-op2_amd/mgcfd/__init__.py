@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
                                   C.POINTER(OpDat), C.c_int, C.POINTER(C.c_double)]
     L.mgcfd_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int)]
     L.mgcfd_plan_stats.argtypes = [OpMap, C.POINTER(C.c_int)]
+    L.mgcfd_part_plan_stats.argtypes = [OpMap, C.POINTER(C.c_int)]
     L.mgcfd_mesh_box.argtypes = [C.c_int, C.c_int, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
     L.mgcfd_mesh_knn.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.POINTER(MeshC))]
     L.mgcfd_mesh_free.argtypes = [C.POINTER(MeshC)]
