@@ -625,22 +625,18 @@ __device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, con
     }
 }
 
-// LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
-// accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
-// precision (records, weights and increment arithmetic in FP32, each
-// increment converted to FP64 and accumulated in FP64 from the FP64 flux).
-// One workgroup per part; NPT staged nodes per thread (NPT * T >= rec_cap).
+// Stages one part and accumulates its local edges: returns the FP64
+// accumulators (in LDS, after a barrier) holding the edge flux of the part's
+// owned nodes, started from the flux dat or, with fresh, from zero.
 template <typename R, int T, int EPT, int NPT, bool COLOURED>
-__global__ __launch_bounds__(T) void k_flux_part(
-    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
-    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
-    double* __restrict__ flux, Consts k, const int* __restrict__ status, int fresh)
+__device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int* __restrict__ halo,
+                                                   const unsigned* __restrict__ ecode, const R* __restrict__ ew,
+                                                   long long nloc, int rec_cap, const double* __restrict__ var,
+                                                   const double* __restrict__ flux, const Consts& k, int fresh,
+                                                   double* sm)
 {
     constexpr int RS = PartRec<R>::kStride;
     constexpr int FL = (5 * kPart + T - 1) / T;
-    extern __shared__ double sm[];
-    if (halted(status)) return;
-    const PartInfo P = parts[xcd_block()];
     const int t = threadIdx.x;
     R* rec = reinterpret_cast<R*>(sm);
     double* acc = sm + ((size_t)rec_cap * RS * sizeof(R) + sizeof(double) - 1) / sizeof(double);
@@ -678,9 +674,28 @@ __global__ __launch_bounds__(T) void k_flux_part(
         part_edges_atomic<R, EPT, RS>(rec, acc, code, w0, w1, w2, sc);
         __syncthreads();
     }
+    return acc;
+}
+
+// LDS: R records [rec_cap * 6] (rounded up to doubles), then FP64
+// accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
+// precision (records, weights and increment arithmetic in FP32, each
+// increment converted to FP64 and accumulated in FP64 from the FP64 flux).
+// One workgroup per part; NPT staged nodes per thread (NPT * T >= rec_cap).
+template <typename R, int T, int EPT, int NPT, bool COLOURED>
+__global__ __launch_bounds__(T) void k_flux_part(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
+    double* __restrict__ flux, Consts k, const int* __restrict__ status, int fresh)
+{
+    extern __shared__ double sm[];
+    if (halted(status)) return;
+    const PartInfo P = parts[xcd_block()];
+    const double* acc =
+        part_accumulate<R, T, EPT, NPT, COLOURED>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh, sm);
     // 4. store
     double* fw = flux + (size_t)P.n0 * 5;
-    for (int j = t; j < P.cnt * 5; j += T) fw[j] = acc[j];
+    for (int j = threadIdx.x; j < P.cnt * 5; j += T) fw[j] = acc[j];
 }
 
 // Local-edge weights of the mixed-precision part kernel, in FP32.
@@ -915,9 +930,10 @@ __global__ void k_copy(size_t n, const double* __restrict__ src, double* __restr
 }
 
 // Block-wide reductions over a 256-thread workgroup (4 waves of 64).
+template <int NT = kBlock>
 __device__ __forceinline__ double block_min(double x)
 {
-    __shared__ double sm[kBlock / kWave];
+    __shared__ double sm[NT / kWave];
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) {
         const double y = __shfl_down(x, o, kWave);
@@ -928,25 +944,27 @@ __device__ __forceinline__ double block_min(double x)
     __syncthreads();
     if (threadIdx.x == 0) {
         x = sm[0];
-        for (int i = 1; i < kBlock / kWave; i++) x = (sm[i] < x) ? sm[i] : x;
+        for (int i = 1; i < NT / kWave; i++) x = (sm[i] < x) ? sm[i] : x;
     }
     return x;
 }
 
 // block_min with the result in every thread.
+template <int NT = kBlock>
 __device__ __forceinline__ double block_min_all(double x)
 {
     __shared__ double bc;
-    x = block_min(x);
+    x = block_min<NT>(x);
     if (threadIdx.x == 0) bc = x;
     __syncthreads();
     return bc;
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ double block_sum(double x)
 {
 #pragma clang fp contract(off)
-    __shared__ double sm[kBlock / kWave];
+    __shared__ double sm[NT / kWave];
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_down(x, o, kWave);
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
@@ -954,14 +972,15 @@ __device__ __forceinline__ double block_sum(double x)
     __syncthreads();
     if (threadIdx.x == 0) {
         x = sm[0];
-        for (int i = 1; i < kBlock / kWave; i++) x += sm[i];
+        for (int i = 1; i < NT / kWave; i++) x += sm[i];
     }
     return x;
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ int block_sum_i(int x)
 {
-    __shared__ int smi[kBlock / kWave];
+    __shared__ int smi[NT / kWave];
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_down(x, o, kWave);
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
@@ -969,7 +988,7 @@ __device__ __forceinline__ int block_sum_i(int x)
     __syncthreads();
     if (threadIdx.x == 0) {
         x = smi[0];
-        for (int i = 1; i < kBlock / kWave; i++) x += smi[i];
+        for (int i = 1; i < NT / kWave; i++) x += smi[i];
     }
     return x;
 }
@@ -1132,9 +1151,115 @@ struct StageArgs {
     const int* status;
 };
 
-__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
+// First stage: the minimum dt of the workgroup's mesh, from the k_calc_dt
+// per-workgroup minima (min is exact, so every workgroup gets the same
+// value); the mesh's first workgroup publishes it and raises abort code 1 on
+// a negative value (euler3d.cpp:476-484).
+template <int NT>
+__device__ __forceinline__ double stage_min_dt(const StageArgs& a, int seg, bool first)
+{
+    const int lo = a.blk ? a.sblk[seg] : 0, hi = a.blk ? a.sblk[seg + 1] : a.np;
+    double m = DBL_MAX;
+    for (int j = lo + (int)threadIdx.x; j < hi; j += NT) m = (a.mpart[j] < m) ? a.mpart[j] : m;
+    m = block_min_all<NT>(m);
+    const double mdt = (m < DBL_MAX) ? m : DBL_MAX;
+    if (first && threadIdx.x == 0) {
+        a.mout[seg] = mdt;
+        if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
+        if (a.zero) *a.zero = 0;
+    }
+    return mdt;
+}
+
+// One node of a Runge-Kutta stage, in the reference's order: f holds node
+// i's edge flux; the node's boundary and far-field increments are added
+// (flux.h:14-39, its state read from vin), the step factor is formed on the
+// first stage (time_stepping_kernels.h:43-64), the update written to a.var
+// (:66-86), the flux reset when asked, and on the last stage the residual
+// with its square sum and NaN/Inf count (validation.h:27-44,102-115).
+// o: node i's old state; vs: its volume on the first stage, else its step
+// factor (loaded by the caller, which may issue the loads early).
+__device__ __forceinline__ void stage_node(const StageArgs& a, const Consts& k0, int seg, int i,
+                                           const double* __restrict__ vin, double f[5], double mdt,
+                                           const double o[5], double vs, double& s, int& nb)
 {
 #pragma clang fp contract(off)
+    double u[5];
+    if (a.bptr) {
+        const int q0 = a.bptr[i], q1 = a.bptr[i + 1];
+        if (q0 < q1) {
+            const Consts& k = a.kseg ? a.kseg[seg] : k0;
+            double U[5];
+            load5(vin, i, U);
+            Th b;
+            thermo_exact(U, b);
+            for (int q = q0; q < q1; q++) {
+                const int e = a.bent[q];
+                const int g = a.bgroup[e];
+                const double w0 = a.bw[3 * (size_t)e], w1 = a.bw[3 * (size_t)e + 1], w2 = a.bw[3 * (size_t)e + 2];
+                if (g <= 2) {
+                    f[0] = f[0] + 0.0;
+                    f[1] = f[1] + w0 * b.p;
+                    f[2] = f[2] + w1 * b.p;
+                    f[3] = f[3] + w2 * b.p;
+                    f[4] = f[4] + 0.0;
+                } else if (g <= 7) {
+                    const double fx = 0.5 * w0, fy = 0.5 * w1, fz = 0.5 * w2;
+                    f[0] += fx * (k.ffv[1] + b.mx) + fy * (k.ffv[2] + b.my) + fz * (k.ffv[3] + b.mz);
+                    f[4] += fx * (k.ffde[0] + b.fdx) + fy * (k.ffde[1] + b.fdy) + fz * (k.ffde[2] + b.fdz);
+                    f[1] += fx * (k.ffmx[0] + b.fxx) + fy * (k.ffmx[1] + b.fxy) + fz * (k.ffmx[2] + b.fxz);
+                    f[2] += fx * (k.ffmy[0] + b.fxy) + fy * (k.ffmy[1] + b.fyy) + fz * (k.ffmy[2] + b.fyz);
+                    f[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
+                }
+            }
+        }
+    }
+    double sfi;
+    if (a.rk == 0) {
+        sfi = mdt / vs;
+        a.sf[i] = sfi;
+    } else {
+        sfi = vs;
+    }
+    const double factor = sfi / (double)(3 + 1 - a.rk);
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = o[v] + factor * f[v];
+    store5(a.var, i, u);
+    if (a.zero_flux) {
+        const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        store5(a.flux, i, z);
+    }
+    if (a.res) {
+        double r[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            r[v] = u[v] - o[v];
+            s += r[v] * r[v];
+            nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
+        }
+        store5(a.res, i, r);
+    }
+}
+
+// The last stage's per-workgroup square sum and NaN/Inf count (slot w).
+template <int NT>
+__device__ __forceinline__ void stage_sums(const StageArgs& a, int w, double s, int nb)
+{
+    if (a.part) {
+        const double t = block_sum<NT>(s);
+        if (threadIdx.x == 0) a.part[w] = t;
+    }
+    if (a.bad || a.badpart) {
+        const int t = block_sum_i<NT>(nb);
+        if (threadIdx.x == 0) {
+            if (a.badpart) a.badpart[w] = t;
+            else if (t) atomicAdd(a.bad, t);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
+{
     if (halted(a.status)) return;
     int i, lim, seg = 0;
     bool first;                // first workgroup of its mesh
@@ -1151,90 +1276,64 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
     }
     double mdt = 0.0;
     if (a.rk == 0) {
-        const int lo = a.blk ? a.sblk[seg] : 0, hi = a.blk ? a.sblk[seg + 1] : a.np;
-        double m = DBL_MAX;
-        for (int j = lo + threadIdx.x; j < hi; j += kBlock) m = (a.mpart[j] < m) ? a.mpart[j] : m;
-        m = block_min_all(m);
-        mdt = (m < DBL_MAX) ? m : DBL_MAX;
-        if (first && threadIdx.x == 0) {
-            a.mout[seg] = mdt;
-            if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
-            if (a.zero) *a.zero = 0;
-        }
+        mdt = stage_min_dt<kBlock>(a, seg, first);
         if (mdt < 0.0 && a.status) return;   // a halting context aborts before any update
     }
     double s = 0.0;
     int nb = 0;
     if (i < lim) {
-        double f[5], o[5], u[5];
+        double f[5], o[5];
         load5(a.flux, i, f);
-        if (a.bptr) {
-            const int q0 = a.bptr[i], q1 = a.bptr[i + 1];
-            if (q0 < q1) {
-                const Consts& k = a.kseg ? a.kseg[seg] : k0;
-                double U[5];
-                load5(a.var, i, U);
-                Th b;
-                thermo_exact(U, b);
-                for (int q = q0; q < q1; q++) {
-                    const int e = a.bent[q];
-                    const int g = a.bgroup[e];
-                    const double w0 = a.bw[3 * (size_t)e], w1 = a.bw[3 * (size_t)e + 1], w2 = a.bw[3 * (size_t)e + 2];
-                    if (g <= 2) {
-                        f[0] = f[0] + 0.0;
-                        f[1] = f[1] + w0 * b.p;
-                        f[2] = f[2] + w1 * b.p;
-                        f[3] = f[3] + w2 * b.p;
-                        f[4] = f[4] + 0.0;
-                    } else if (g <= 7) {
-                        const double fx = 0.5 * w0, fy = 0.5 * w1, fz = 0.5 * w2;
-                        f[0] += fx * (k.ffv[1] + b.mx) + fy * (k.ffv[2] + b.my) + fz * (k.ffv[3] + b.mz);
-                        f[4] += fx * (k.ffde[0] + b.fdx) + fy * (k.ffde[1] + b.fdy) + fz * (k.ffde[2] + b.fdz);
-                        f[1] += fx * (k.ffmx[0] + b.fxx) + fy * (k.ffmx[1] + b.fxy) + fz * (k.ffmx[2] + b.fxz);
-                        f[2] += fx * (k.ffmy[0] + b.fxy) + fy * (k.ffmy[1] + b.fyy) + fz * (k.ffmy[2] + b.fyz);
-                        f[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
-                    }
-                }
-            }
-        }
-        double sfi;
-        if (a.rk == 0) {
-            sfi = mdt / a.vol[i];
-            a.sf[i] = sfi;
-        } else {
-            sfi = a.sf[i];
-        }
-        const double factor = sfi / (double)(3 + 1 - a.rk);
         load5(a.old, i, o);
+        stage_node(a, k0, seg, i, a.var, f, mdt, o, a.rk == 0 ? a.vol[i] : a.sf[i], s, nb);
+    }
+    stage_sums<kBlock>(a, blockIdx.x, s, nb);
+}
+
+// The FP64 / mixed part kernel fused with its Runge-Kutta stage (device
+// multigrid driver): a part's accumulators hold the complete edge flux of its
+// owned nodes, so the stage update of those nodes runs in the same
+// workgroup.  Other parts read the stage's input state as their halo while
+// this one writes its update, so the input (vin) and the update (a.var) are
+// different buffers.  The edge flux starts from the flux dat on the first
+// stage (fresh == 0) and from zero after it; it is not written back.
+// pseg / pfirst: segment of each part and first part of each segment
+// (batched solves), null for one mesh.
+template <typename R, int T, int EPT, int NPT, bool COLOURED>
+__global__ __launch_bounds__(T) void k_flux_part_stage(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
+    StageArgs a, Consts k0, const int* __restrict__ pseg, const int* __restrict__ pfirst)
+{
+    extern __shared__ double sm[];
+    if (halted(a.status)) return;
+    const int p = xcd_block();
+    const PartInfo P = parts[p];
+    const int seg = pseg ? pseg[p] : 0;
+    static_assert(T >= kPart, "one owned node per thread");
+    double mdt = 0.0;
+    if (a.rk == 0) {
+        mdt = stage_min_dt<T>(a, seg, pseg ? p == pfirst[seg] : p == 0);
+        if (mdt < 0.0 && a.status) return;
+    }
+    // the owned node's old state and volume / step factor, in flight through
+    // the edge phase (clamped to a valid node for threads without one)
+    const int t = threadIdx.x;
+    const int i = P.n0 + min(t, max(P.cnt - 1, 0));
+    double o[5];
+    load5(a.old, i, o);
+    const double vs = a.rk == 0 ? a.vol[i] : a.sf[i];
+    double* acc = part_accumulate<R, T, EPT, NPT, COLOURED>(P, halo, ecode, ew, nloc, rec_cap, vin, a.flux,
+                                                            k0, fresh, sm);
+    double s = 0.0;
+    int nb = 0;
+    if (t < P.cnt) {
+        double f[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) u[v] = o[v] + factor * f[v];
-        store5(a.var, i, u);
-        if (a.zero_flux) {
-            const double z[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            store5(a.flux, i, z);
-        }
-        if (a.res) {
-            double r[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                r[v] = u[v] - o[v];
-                s += r[v] * r[v];
-                nb += (isnan(u[v]) || isinf(u[v])) ? 1 : 0;
-            }
-            store5(a.res, i, r);
-        }
+        for (int v = 0; v < 5; v++) f[v] = acc[t * 5 + v];
+        stage_node(a, k0, seg, i, vin, f, mdt, o, vs, s, nb);
     }
-    if (a.part) {
-        const double t = block_sum(s);
-        if (threadIdx.x == 0) a.part[blockIdx.x] = t;
-    }
-    if (a.bad || a.badpart) {
-        const int t = block_sum_i(nb);
-        if (threadIdx.x == 0) {
-            if (a.badpart) a.badpart[blockIdx.x] = t;
-            else if (t) atomicAdd(a.bad, t);
-        }
-    }
+    stage_sums<T>(a, p, s, nb);
 }
 
 // residual_kernel (validation.h:27-35); with 'part' set also the per-node

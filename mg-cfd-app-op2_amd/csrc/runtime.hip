@@ -100,6 +100,8 @@ struct InvPlan {
     long long pt_nloc = 0;       // local edges over all parts
     int pt_rec_cap = 0;          // max staged nodes (owned + halo) of one part
     int pt_rec_med = 0;          // median staged nodes of a part
+    std::vector<PartInfo> h_pt_info;   // host copy of the parts
+    unsigned pt_gen = 0;         // identifies the part plan built last (process-wide count)
     int pt_max_ne = 0;           // max local edges of one part
     int pt_max_col = 0;
     size_t pt_lds = 0;
@@ -877,6 +879,9 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     TRY(up(P.d_pt_eid, eid));
     HIPCHK(hipMalloc(&P.d_pt_w, sizeof(double) * 3 * std::max<size_t>(1, eid.size())));
     P.pt_parts = (int)parts.size();
+    P.h_pt_info = parts;
+    static std::atomic<unsigned> gen{0};
+    P.pt_gen = ++gen;
     P.pt_nloc = (long long)eid.size();
     P.pt_rec_cap = rec_cap;
     {
@@ -974,6 +979,7 @@ static void free_partplan(InvPlan& P)
     P.d_pt_info = nullptr; P.d_pt_halo = nullptr; P.d_pt_code = nullptr; P.d_pt_eid = nullptr;
     P.d_pt_w = nullptr; P.d_pt_w32 = nullptr;
     P.pt_parts = 0;
+    P.h_pt_info.clear();
     P.pt_wdat = P.pt_w32dat = nullptr;
     P.pt_wver = P.pt_w32ver = 0;
 }
@@ -1180,10 +1186,30 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
     return check_launch();
 }
 
+// The stage fused into a part kernel (k_flux_part_stage): its input state,
+// stage arguments and, for batched solves, the part -> segment tables.
+struct PartStage {
+    const double* vin = nullptr;
+    const StageArgs* a = nullptr;
+    const int* pseg = nullptr;
+    const int* pfirst = nullptr;
+};
+
 template <typename Real, int T, int EPT, int NPT>
 static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                         const int* status, int fresh)
+                         const int* status, int fresh, const PartStage* st)
 {
+    if (st) {
+        if (coloured)
+            k_flux_part_stage<Real, T, EPT, NPT, true><<<P.pt_parts, T, lds, R.stream>>>(
+                P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
+                st->pseg, st->pfirst);
+        else
+            k_flux_part_stage<Real, T, EPT, NPT, false><<<P.pt_parts, T, lds, R.stream>>>(
+                P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
+                st->pseg, st->pfirst);
+        return check_launch();
+    }
     if (coloured)
         k_flux_part<Real, T, EPT, NPT, true><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
@@ -1199,13 +1225,13 @@ static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t 
 // largest part allows it.
 template <typename Real, int T, int EPT>
 static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                         const int* status, int fresh)
+                         const int* status, int fresh, const PartStage* st)
 {
     constexpr int kSmall = (512 + T - 1) / T, kLarge = (768 + T - 1) / T;
     static_assert(kPartSlots <= kLarge * T, "staged nodes exceed the kernel's registers");
     if (P.pt_rec_cap <= kSmall * T)
-        return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status, fresh);
-    return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status, fresh);
+        return launch_part_t<Real, T, EPT, kSmall>(coloured, P, w, lds, var, flux, status, fresh, st);
+    return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status, fresh, st);
 }
 
 // Real = double: the FP64 part kernel, 256 threads per part; Real = float:
@@ -1213,14 +1239,14 @@ static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t 
 // waves) share one part, which shortens each workgroup's edge phase.
 template <typename Real>
 static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                       const int* status, int fresh)
+                       const int* status, int fresh, const PartStage* st = nullptr)
 {
     constexpr int T = sizeof(Real) == 4 ? 512 : 256;
     const int ept = (P.pt_max_ne + T - 1) / T;
-    if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status, fresh);
-    if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status, fresh);
-    if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status, fresh);
-    if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status, fresh);
+    if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
+    if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
+    if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
+    if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
     return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 3072);
 }
 
@@ -1235,6 +1261,22 @@ static int prepare_flux(int variant, op_map map, op_dat w)
     InvPlan& P = M(map)->inv;
     TRY(refresh_plan_weights(variant, P, w));
     return 0;
+}
+
+// Edge flux of a part variant fused with a Runge-Kutta stage (device
+// multigrid driver); plans and weights come from prepare_flux.
+static bool fused_stage_variant(int v) { return part_variant(v) || mixed_variant(v); }
+
+static int launch_flux_stage(int variant, op_map map, const PartStage& st, int fresh)
+{
+    const InvPlan& P = M(map)->inv;
+    if (!(part_variant(variant) || mixed_variant(variant)) || P.pt_parts == 0)
+        return fail("edge-flux variant %d has no part plan for map %s", variant, map->name);
+    if (mixed_variant(variant))
+        return launch_part<float>(variant == MGCFD_FLUX_MIXED_COLOURED, P, P.d_pt_w32, P.pt_lds32, nullptr, nullptr,
+                                  st.a->status, fresh, &st);
+    return launch_part<double>(variant == MGCFD_FLUX_PART_COLOURED, P, P.d_pt_w, P.pt_lds, nullptr, nullptr,
+                               st.a->status, fresh, &st);
 }
 
 // fresh: the edge fluxes start from zero instead of adding to the flux dat

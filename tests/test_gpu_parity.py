@@ -437,6 +437,20 @@ def test_solve_parity(rt, orc, meshname, mode, request):
     assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
 
 
+@pytest.mark.parametrize("variant", ["part_coloured", "part_atomic"])
+@pytest.mark.parametrize("mode", ["device", "graph"])
+def test_solve_parity_part_variants(rt, orc, knn_small, variant, mode):
+    """In the device driver the part variants run each Runge-Kutta stage
+    inside the edge-flux kernel, reading one state buffer and writing another;
+    the solve still matches the oracle at every level."""
+    ref = orc.solve(knn_small, 3, 0.5, 3.0)
+    st, rms, states = solve_gpu(rt, knn_small, 3, 0.5, 3.0, mode, variant=variant)
+    assert st == 0
+    for l in range(len(knn_small)):
+        assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
+    assert np.allclose(rms, ref["rms"], rtol=1e-10, atol=0)
+
+
 def test_solve_parity_down_v2(rt, orc, knn_small):
     """A 3-cycle solve prolonging with the inverse-distance loops, against
     the oracle's run of the same loop order (euler3d.cpp:603-623)."""
@@ -536,7 +550,8 @@ def test_mixed_solve_error(rt, orc, knn_small, variant):
     assert rms_err.max() <= 1e-3
 
 
-def test_bad_value_abort(rt, orc, box8):
+@pytest.mark.parametrize("variant", ["gather_exact", "part_atomic", "mixed_atomic"])
+def test_bad_value_abort(rt, orc, box8, variant):
     """NaN injected after setup: abort code 2 after the first L0 step
     (euler3d.cpp:539-548), same as the oracle, and the state stops there."""
     def poison(arrs):
@@ -544,6 +559,7 @@ def test_bad_value_abort(rt, orc, box8):
     ref = orc.solve(box8, 3, 0.5, 3.0, after_setup=poison)
     assert ref["status"] == 2 and ref["steps"] == 1
     s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
     s.setup()
     v = s.state(0)
     v[3, 1] = np.nan
@@ -555,4 +571,27 @@ def test_bad_value_abort(rt, orc, box8):
     ok = np.isnan(g) == np.isnan(o)
     assert ok.all()
     m = ~np.isnan(o)
-    assert rel_err(np.where(m, g, 0), np.where(m, o, 0)) <= STATE_TOL
+    tol = MIXED_STATE_TOL if variant.startswith("mixed") else STATE_TOL
+    assert rel_err(np.where(m, g, 0), np.where(m, o, 0)) <= tol
+
+
+@pytest.mark.parametrize("variant", ["gather_exact", "part_atomic", "mixed_coloured"])
+def test_negative_dt_abort(rt, orc, box8, variant):
+    """A negative cell volume gives a negative min dt: abort code 1 before
+    any update (euler3d.cpp:480-484), as in the oracle, in every driver
+    form of the first stage."""
+    def poison(arrs):
+        arrs[0]["vol"][5] = -1.0
+    ref = orc.solve(box8, 3, 0.5, 3.0, after_setup=poison)
+    assert ref["status"] == 1 and ref["steps"] == 1
+    s = mgcfd.Solver(rt, box8, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.setup()
+    v0 = s.state(0)
+    vol = s.state(0, "volumes")
+    vol[5, 0] = -1.0
+    rt.upload(s.lv[0].volumes, vol)
+    st, rms = s.run_device(3, use_graph=True)
+    assert st == 1
+    assert bitwise(s.state(0), v0)
+    assert bitwise(v0, ref["levels"][0]["variables"]) or rel_err(v0, ref["levels"][0]["variables"]) <= STATE_TOL
