@@ -1138,10 +1138,8 @@ struct StageArgs {
     double* flux;
     const double* old;
     double* var;
-    const int* bptr;           // boundary elements of node i: bent[bptr[i] .. bptr[i + 1]) (null: none)
-    const int* bent;
-    const int* bgroup;
-    const double* bw;
+    const int* bptr;           // boundary elements of node i: bnw[bptr[i] .. bptr[i + 1]) (null: none)
+    const double4* bnw;        // per boundary element in that order: {w0, w1, w2, group}
     const Consts* kseg;        // batched solves: far-field constants per segment
     double* res;               // last stage: residual, or null
     double* part;              // last stage, finest level: per-workgroup sum of squares
@@ -1178,39 +1176,37 @@ __device__ __forceinline__ double stage_min_dt(const StageArgs& a, int seg, bool
 // (:66-86), the flux reset when asked, and on the last stage the residual
 // with its square sum and NaN/Inf count (validation.h:27-44,102-115).
 // o: node i's old state; vs: its volume on the first stage, else its step
-// factor (loaded by the caller, which may issue the loads early).
+// factor; [q0, q1): its boundary elements (all loaded by the caller, which
+// may issue the loads early).
 __device__ __forceinline__ void stage_node(const StageArgs& a, const Consts& k0, int seg, int i,
                                            const double* __restrict__ vin, double f[5], double mdt,
-                                           const double o[5], double vs, double& s, int& nb)
+                                           const double o[5], double vs, int q0, int q1, double& s, int& nb)
 {
 #pragma clang fp contract(off)
     double u[5];
-    if (a.bptr) {
-        const int q0 = a.bptr[i], q1 = a.bptr[i + 1];
-        if (q0 < q1) {
-            const Consts& k = a.kseg ? a.kseg[seg] : k0;
-            double U[5];
-            load5(vin, i, U);
-            Th b;
-            thermo_exact(U, b);
-            for (int q = q0; q < q1; q++) {
-                const int e = a.bent[q];
-                const int g = a.bgroup[e];
-                const double w0 = a.bw[3 * (size_t)e], w1 = a.bw[3 * (size_t)e + 1], w2 = a.bw[3 * (size_t)e + 2];
-                if (g <= 2) {
-                    f[0] = f[0] + 0.0;
-                    f[1] = f[1] + w0 * b.p;
-                    f[2] = f[2] + w1 * b.p;
-                    f[3] = f[3] + w2 * b.p;
-                    f[4] = f[4] + 0.0;
-                } else if (g <= 7) {
-                    const double fx = 0.5 * w0, fy = 0.5 * w1, fz = 0.5 * w2;
-                    f[0] += fx * (k.ffv[1] + b.mx) + fy * (k.ffv[2] + b.my) + fz * (k.ffv[3] + b.mz);
-                    f[4] += fx * (k.ffde[0] + b.fdx) + fy * (k.ffde[1] + b.fdy) + fz * (k.ffde[2] + b.fdz);
-                    f[1] += fx * (k.ffmx[0] + b.fxx) + fy * (k.ffmx[1] + b.fxy) + fz * (k.ffmx[2] + b.fxz);
-                    f[2] += fx * (k.ffmy[0] + b.fxy) + fy * (k.ffmy[1] + b.fyy) + fz * (k.ffmy[2] + b.fyz);
-                    f[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
-                }
+    if (q0 < q1) {
+        const Consts& k = a.kseg ? a.kseg[seg] : k0;
+        double U[5];
+        load5(vin, i, U);
+        Th b;
+        thermo_exact(U, b);
+        for (int q = q0; q < q1; q++) {
+            const double4 bq = a.bnw[q];
+            const int g = (int)bq.w;
+            const double w0 = bq.x, w1 = bq.y, w2 = bq.z;
+            if (g <= 2) {
+                f[0] = f[0] + 0.0;
+                f[1] = f[1] + w0 * b.p;
+                f[2] = f[2] + w1 * b.p;
+                f[3] = f[3] + w2 * b.p;
+                f[4] = f[4] + 0.0;
+            } else if (g <= 7) {
+                const double fx = 0.5 * w0, fy = 0.5 * w1, fz = 0.5 * w2;
+                f[0] += fx * (k.ffv[1] + b.mx) + fy * (k.ffv[2] + b.my) + fz * (k.ffv[3] + b.mz);
+                f[4] += fx * (k.ffde[0] + b.fdx) + fy * (k.ffde[1] + b.fdy) + fz * (k.ffde[2] + b.fdz);
+                f[1] += fx * (k.ffmx[0] + b.fxx) + fy * (k.ffmx[1] + b.fxy) + fz * (k.ffmx[2] + b.fxz);
+                f[2] += fx * (k.ffmy[0] + b.fxy) + fy * (k.ffmy[1] + b.fyy) + fz * (k.ffmy[2] + b.fyz);
+                f[3] += fx * (k.ffmz[0] + b.fxz) + fy * (k.ffmz[1] + b.fyz) + fz * (k.ffmz[2] + b.fzz);
             }
         }
     }
@@ -1258,6 +1254,18 @@ __device__ __forceinline__ void stage_sums(const StageArgs& a, int w, double s, 
     }
 }
 
+// The stage's boundary table: the weights and group of each boundary
+// element, in the node -> boundary-element CSR order of 'bent', so a node
+// reads its elements as one contiguous run.
+__global__ void k_pack_bnd(int nq, const int* __restrict__ bent, const double* __restrict__ bw,
+                           const int* __restrict__ bgroup, double4* __restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const size_t e = (size_t)bent[q];
+    out[q] = make_double4(bw[3 * e], bw[3 * e + 1], bw[3 * e + 2], (double)bgroup[e]);
+}
+
 __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 {
     if (halted(a.status)) return;
@@ -1285,7 +1293,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
         double f[5], o[5];
         load5(a.flux, i, f);
         load5(a.old, i, o);
-        stage_node(a, k0, seg, i, a.var, f, mdt, o, a.rk == 0 ? a.vol[i] : a.sf[i], s, nb);
+        const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
+        stage_node(a, k0, seg, i, a.var, f, mdt, o, a.rk == 0 ? a.vol[i] : a.sf[i], q0, q1, s, nb);
     }
     stage_sums<kBlock>(a, blockIdx.x, s, nb);
 }
@@ -1323,6 +1332,7 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
     double o[5];
     load5(a.old, i, o);
     const double vs = a.rk == 0 ? a.vol[i] : a.sf[i];
+    const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
     double* acc = part_accumulate<R, T, EPT, NPT, COLOURED>(P, halo, ecode, ew, nloc, rec_cap, vin, a.flux,
                                                             k0, fresh, sm);
     double s = 0.0;
@@ -1331,7 +1341,7 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
         double f[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) f[v] = acc[t * 5 + v];
-        stage_node(a, k0, seg, i, vin, f, mdt, o, vs, s, nb);
+        stage_node(a, k0, seg, i, vin, f, mdt, o, vs, q0, q1, s, nb);
     }
     stage_sums<T>(a, p, s, nb);
 }
