@@ -443,7 +443,7 @@ def main():
         summarize_pmc(*args.summarize_pmc)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 

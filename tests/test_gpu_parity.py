@@ -20,7 +20,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-STATE_TOL = 1e-12        # state after N cycles, per variable
+STATE_TOL = 1e-12        # state after N cycles, relative per variable
 FLUX_FAST_TOL = 1e-12    # one flux pass, fast / atomic variants
 # Mixed precision (FP32 flux arithmetic, FP64 accumulation).  One flux pass
 # is compared with the FP64 oracle relative to the largest flux of each
