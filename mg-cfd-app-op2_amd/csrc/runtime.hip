@@ -51,7 +51,6 @@ static int fail(const char* fmt, ...)
         if (e_ != hipSuccess) return fail("%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-static inline int round_up(int x, int m) { return ((x + m - 1) / m) * m; }
 static inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // ---------------------------------------------------------------- state --
