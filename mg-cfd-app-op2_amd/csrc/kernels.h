@@ -509,6 +509,28 @@ struct PartRec<float> {
     static __device__ __forceinline__ PState<float> state(const float* r) { return part_state32(r); }
 };
 
+// An endpoint state from a record of RS values: the six-value records above,
+// or (FP64, RS = 5) the raw {rho, m, E}, whose wave speed is formed here.
+// The FP64 kernel takes 40-byte records when 48-byte ones would leave fewer
+// than four workgroups per CU.
+template <typename R, int RS>
+__device__ __forceinline__ PState<R> part_rec_state(const R* r)
+{
+    if constexpr (RS == 5) {
+        static_assert(sizeof(R) == sizeof(double), "40-byte records are FP64");
+        PState<double> t;
+        t.rho = r[0]; t.mx = r[1]; t.my = r[2]; t.mz = r[3]; t.E = r[4];
+        const double inv = part_rcp(t.rho);
+        t.ux = t.mx * inv; t.uy = t.my * inv; t.uz = t.mz * inv;
+        const double ssq = t.ux * t.ux + t.uy * t.uy + t.uz * t.uz;
+        t.p = (kGamma - 1.0) * (t.E - 0.5 * t.rho * ssq);
+        t.s = part_sqrt(ssq) + part_sqrt(kGamma * t.p * inv);
+        return t;
+    } else {
+        return PartRec<R>::state(r);
+    }
+}
+
 // Marks a loaded register as needed here: the compiler waits for its load at
 // this point and cannot sink the load into a later divergent branch (where it
 // would be issued alone under its own wait).
@@ -561,7 +583,7 @@ __device__ __forceinline__ void part_store_records(const PartInfo& P, int t, con
             const R x0 = (R)U[k][0], x1 = (R)U[k][1], x2 = (R)U[k][2], x3 = (R)U[k][3], x4 = (R)U[k][4];
             R* r = rec + sl * RS;
             r[0] = x0; r[1] = x1; r[2] = x2; r[3] = x3; r[4] = x4;
-            r[5] = part_speed<R>(x0, x1, x2, x3, x4);
+            if constexpr (RS > 5) r[5] = part_speed<R>(x0, x1, x2, x3, x4);
         }
     }
 }
@@ -617,8 +639,8 @@ __device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, con
 #pragma unroll
     for (int i = 0; i < EPT; i++) {
         if (!code[i]) continue;
-        const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
-        const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
+        const PState<R> a = part_rec_state<R, RS>(rec + (code[i] & 1023u) * RS);
+        const PState<R> b = part_rec_state<R, RS>(rec + ((code[i] >> 10) & 1023u) * RS);
         R inc[5];
         part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc);
         part_apply(acc, code[i], inc);
@@ -628,14 +650,13 @@ __device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, con
 // Stages one part and accumulates its local edges: returns the FP64
 // accumulators (in LDS, after a barrier) holding the edge flux of the part's
 // owned nodes, started from the flux dat or, with fresh, from zero.
-template <typename R, int T, int EPT, int NPT, bool COLOURED>
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
 __device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int* __restrict__ halo,
                                                    const unsigned* __restrict__ ecode, const R* __restrict__ ew,
                                                    long long nloc, int rec_cap, const double* __restrict__ var,
                                                    const double* __restrict__ flux, const Consts& k, int fresh,
                                                    double* sm)
 {
-    constexpr int RS = PartRec<R>::kStride;
     constexpr int FL = (5 * kPart + T - 1) / T;
     const int t = threadIdx.x;
     R* rec = reinterpret_cast<R*>(sm);
@@ -660,8 +681,8 @@ __device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int*
 #pragma unroll
         for (int i = 0; i < EPT; i++)
             if (code[i]) {
-                const PState<R> a = PartRec<R>::state(rec + (code[i] & 1023u) * RS);
-                const PState<R> b = PartRec<R>::state(rec + ((code[i] >> 10) & 1023u) * RS);
+                const PState<R> a = part_rec_state<R, RS>(rec + (code[i] & 1023u) * RS);
+                const PState<R> b = part_rec_state<R, RS>(rec + ((code[i] >> 10) & 1023u) * RS);
                 part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc[i]);
             }
         for (int c = 0; c < P.ncol; c++) {
@@ -681,8 +702,9 @@ __device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int*
 // accumulators [cnt * 5].  R = double: the FP64 kernel; R = float: mixed
 // precision (records, weights and increment arithmetic in FP32, each
 // increment converted to FP64 and accumulated in FP64 from the FP64 flux).
-// One workgroup per part; NPT staged nodes per thread (NPT * T >= rec_cap).
-template <typename R, int T, int EPT, int NPT, bool COLOURED>
+// One workgroup per part; NPT staged nodes per thread (NPT * T >= rec_cap);
+// RS values per record.
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
 __global__ __launch_bounds__(T) void k_flux_part(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ var,
@@ -692,7 +714,7 @@ __global__ __launch_bounds__(T) void k_flux_part(
     if (halted(status)) return;
     const PartInfo P = parts[xcd_block()];
     const double* acc =
-        part_accumulate<R, T, EPT, NPT, COLOURED>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh, sm);
+        part_accumulate<R, T, EPT, NPT, COLOURED, RS>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh, sm);
     // 4. store
     double* fw = flux + (size_t)P.n0 * 5;
     for (int j = threadIdx.x; j < P.cnt * 5; j += T) fw[j] = acc[j];
@@ -1308,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 // stage (fresh == 0) and from zero after it; it is not written back.
 // pseg / pfirst: segment of each part and first part of each segment
 // (batched solves), null for one mesh.
-template <typename R, int T, int EPT, int NPT, bool COLOURED>
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
 __global__ __launch_bounds__(T) void k_flux_part_stage(
     const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
@@ -1333,8 +1355,8 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
     load5(a.old, i, o);
     const double vs = a.rk == 0 ? a.vol[i] : a.sf[i];
     const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
-    double* acc = part_accumulate<R, T, EPT, NPT, COLOURED>(P, halo, ecode, ew, nloc, rec_cap, vin, a.flux,
-                                                            k0, fresh, sm);
+    double* acc = part_accumulate<R, T, EPT, NPT, COLOURED, RS>(P, halo, ecode, ew, nloc, rec_cap, vin, a.flux,
+                                                                k0, fresh, sm);
     double s = 0.0;
     int nb = 0;
     if (t < P.cnt) {

@@ -99,6 +99,7 @@ struct InvPlan {
     long long pt_nloc = 0;       // local edges over all parts
     int pt_rec_cap = 0;          // max staged nodes (owned + halo) of one part
     int pt_rec_med = 0;          // median staged nodes of a part
+    int pt_rs = 6;               // values per FP64 LDS record (6, or 5 when 48-byte records would cost a workgroup)
     std::vector<PartInfo> h_pt_info;   // host copy of the parts
     unsigned pt_gen = 0;         // identifies the part plan built last (process-wide count)
     int pt_max_ne = 0;           // max local edges of one part
@@ -891,7 +892,10 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     }
     P.pt_max_ne = max_ne;
     P.pt_max_col = max_col;
-    P.pt_lds = sizeof(double) * ((size_t)rec_cap * kPRec + (size_t)kPart * 5);
+    // 48-byte FP64 records while four workgroups fit a CU's 160 KB of LDS,
+    // else 40-byte ones (the wave speed then formed per edge)
+    P.pt_rs = (size_t)rec_cap * kPRec * sizeof(double) + (size_t)kPart * 5 * sizeof(double) > 40960 ? 5 : kPRec;
+    P.pt_lds = sizeof(double) * ((size_t)rec_cap * P.pt_rs + (size_t)kPart * 5);
     P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * PartRec<float>::kStride * sizeof(float) + 7) / 8 +
                                    (size_t)kPart * 5);   // FP32 records, FP64 accumulators
     return 0;
@@ -1194,30 +1198,43 @@ struct PartStage {
     const int* pfirst = nullptr;
 };
 
-template <typename Real, int T, int EPT, int NPT>
-static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                         const int* status, int fresh, const PartStage* st)
+template <typename Real, int T, int EPT, int NPT, int RS>
+static int launch_part_rs(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                          const int* status, int fresh, const PartStage* st)
 {
     if (st) {
         if (coloured)
-            k_flux_part_stage<Real, T, EPT, NPT, true><<<P.pt_parts, T, lds, R.stream>>>(
+            k_flux_part_stage<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, R.stream>>>(
                 P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                 st->pseg, st->pfirst);
         else
-            k_flux_part_stage<Real, T, EPT, NPT, false><<<P.pt_parts, T, lds, R.stream>>>(
+            k_flux_part_stage<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, R.stream>>>(
                 P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                 st->pseg, st->pfirst);
         return check_launch();
     }
     if (coloured)
-        k_flux_part<Real, T, EPT, NPT, true><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     else
-        k_flux_part<Real, T, EPT, NPT, false><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, R.stream>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     return check_launch();
+}
+
+// Record size: FP32 records have six values; FP64 ones the plan's pt_rs.
+template <typename Real, int T, int EPT, int NPT>
+static int launch_part_t(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                         const int* status, int fresh, const PartStage* st)
+{
+    if constexpr (sizeof(Real) == sizeof(double)) {
+        if (P.pt_rs == 5)
+            return launch_part_rs<Real, T, EPT, NPT, 5>(coloured, P, w, lds, var, flux, status, fresh, st);
+    }
+    return launch_part_rs<Real, T, EPT, NPT, PartRec<Real>::kStride>(coloured, P, w, lds, var, flux, status, fresh,
+                                                                    st);
 }
 
 // Staged nodes per thread: the smaller register array when the part plan's

@@ -595,3 +595,47 @@ def test_negative_dt_abort(rt, orc, box8, variant):
     assert st == 1
     assert bitwise(s.state(0), v0)
     assert bitwise(v0, ref["levels"][0]["variables"]) or rel_err(v0, ref["levels"][0]["variables"]) <= STATE_TOL
+
+
+@pytest.fixture(scope="module")
+def knn_dense():
+    """A kNN mesh of degree ~14, whose 256-node parts stage so many halo
+    nodes that 48-byte FP64 records would leave fewer than four workgroups
+    per CU: its part plans take 40-byte records."""
+    return mgcfd.mesh_knn(12000, 84000, 2, seed=5)
+
+
+def test_part_kernels_with_40_byte_records(rt, orc, knn_dense):
+    mesh = knn_dense
+    s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
+    s.setup()
+    rs5 = []
+    for l in range(len(mesh)):
+        st = (C.c_int * 6)()
+        assert rt.L.mgcfd_part_plan_stats(s.lv[l].edge_to_node, st) == 0
+        rec_max, lds = st[1], st[5]
+        if rec_max * 48 + 256 * 40 > 40960:
+            assert lds == rec_max * 40 + 256 * 40, (l, rec_max, lds)
+            rs5.append(l)
+        else:
+            assert lds == rec_max * 48 + 256 * 40, (l, rec_max, lds)
+    assert rs5, "no level of the dense mesh takes 40-byte records"
+    c = orc.freestream(0.5, 3.0)
+    ref = oracle_setup(orc, mesh, c)
+    for l in rs5:
+        n = len(mesh[l]["coords"])
+        var = perturbed_state(n, c.ff_variable[:], seed=31 + l)
+        for variant in ("part_atomic", "part_coloured"):
+            rt.upload(s.lv[l].variables, var)
+            rt.upload(s.lv[l].fluxes, np.zeros((n, 5)))
+            rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+            s.flux_edge(l)
+            o = np.zeros((n, 5))
+            orc.compute_flux_edge(np.ascontiguousarray(mesh[l]["edge_to_node"], np.int32), var, ref[l]["ew"], o, c)
+            assert rel_err(s.state(l, "fluxes"), o) <= FLUX_FAST_TOL, (l, variant)
+    # and a device solve through the fused stage kernel
+    ref = orc.solve(mesh, 2, 0.5, 3.0)
+    st2, rms, states = solve_gpu(rt, mesh, 2, 0.5, 3.0, "graph", variant="part_atomic")
+    assert st2 == 0
+    for l in range(len(mesh)):
+        assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
