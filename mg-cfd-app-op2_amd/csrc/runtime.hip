@@ -99,7 +99,7 @@ struct InvPlan {
     long long pt_nloc = 0;       // local edges over all parts
     int pt_rec_cap = 0;          // max staged nodes (owned + halo) of one part
     int pt_rec_med = 0;          // median staged nodes of a part
-    int pt_rs = 6;               // values per FP64 LDS record (6, or 5 when 48-byte records would cost a workgroup)
+    int pt_rs = 6;               // values per FP64 LDS record (6, or 5: see build_partplan)
     std::vector<PartInfo> h_pt_info;   // host copy of the parts
     unsigned pt_gen = 0;         // identifies the part plan built last (process-wide count)
     int pt_max_ne = 0;           // max local edges of one part
@@ -143,6 +143,7 @@ struct DatImpl {
 struct Runtime {
     bool inited = false;
     int device = 0;
+    int n_cu = 256;              // compute units of the device
     hipStream_t stream = nullptr;
     int base_index = 0;
     bool frozen = false;         // at least one loop has run
@@ -892,9 +893,13 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     }
     P.pt_max_ne = max_ne;
     P.pt_max_col = max_col;
-    // 48-byte FP64 records while four workgroups fit a CU's 160 KB of LDS,
-    // else 40-byte ones (the wave speed then formed per edge)
-    P.pt_rs = (size_t)rec_cap * kPRec * sizeof(double) + (size_t)kPart * 5 * sizeof(double) > 40960 ? 5 : kPRec;
+    // FP64 records: 40 bytes (the wave speed formed per edge) where 48-byte
+    // ones would leave fewer than four workgroups per CU, and where the parts
+    // run as one round (fewer than the device's 4 workgroups per CU): there a
+    // dispatch lasts one workgroup lifetime, which staging no wave speeds
+    // shortens; else 48 bytes (the speed formed once per staged node).
+    const bool lds_bound = (size_t)rec_cap * kPRec * sizeof(double) + (size_t)kPart * 5 * sizeof(double) > 40960;
+    P.pt_rs = (lds_bound || (long long)parts.size() < 4LL * R.n_cu) ? 5 : kPRec;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * P.pt_rs + (size_t)kPart * 5);
     P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * PartRec<float>::kStride * sizeof(float) + 7) / 8 +
                                    (size_t)kPart * 5);   // FP32 records, FP64 accumulators
@@ -1384,6 +1389,11 @@ int op_init(int argc, char** argv, int diags)
     if (ndev <= 0) return fail("op_init: no HIP device visible");
     if (R.device >= ndev) return fail("op_init: device %d not present (%d visible)", R.device, ndev);
     HIPCHK(hipSetDevice(R.device));
+    {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, R.device));
+        R.n_cu = std::max(1, prop.multiProcessorCount);
+    }
     HIPCHK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&R.d_scal, 4 * sizeof(double)));
     HIPCHK(hipMalloc(&R.d_iscal, 4 * sizeof(int)));

@@ -601,30 +601,31 @@ def test_negative_dt_abort(rt, orc, box8, variant):
 def knn_dense():
     """A kNN mesh of degree ~14, whose 256-node parts stage so many halo
     nodes that 48-byte FP64 records would leave fewer than four workgroups
-    per CU: its part plans take 40-byte records."""
+    per CU."""
     return mgcfd.mesh_knn(12000, 84000, 2, seed=5)
 
 
-def test_part_kernels_with_40_byte_records(rt, orc, knn_dense):
-    mesh = knn_dense
+def check_part_records(rt, orc, mesh, seed):
+    """Each level's part plan takes 40-byte FP64 records where 48-byte ones
+    would leave fewer than four workgroups per CU or its parts do not fill
+    the device's workgroup slots (4 per CU), else 48-byte ones; the FP64 part
+    kernels match the oracle on every level.  Returns the record sizes."""
+    name, ncu = C.create_string_buffer(256), C.c_int(0)
+    assert rt.L.mgcfd_device_info(name, 256, C.byref(ncu)) == 0
     s = mgcfd.Solver(rt, mesh, 0.5, 3.0)
     s.setup()
-    rs5 = []
+    c = orc.freestream(0.5, 3.0)
+    ref = oracle_setup(orc, mesh, c)
+    sizes = []
     for l in range(len(mesh)):
         st = (C.c_int * 6)()
         assert rt.L.mgcfd_part_plan_stats(s.lv[l].edge_to_node, st) == 0
-        rec_max, lds = st[1], st[5]
-        if rec_max * 48 + 256 * 40 > 40960:
-            assert lds == rec_max * 40 + 256 * 40, (l, rec_max, lds)
-            rs5.append(l)
-        else:
-            assert lds == rec_max * 48 + 256 * 40, (l, rec_max, lds)
-    assert rs5, "no level of the dense mesh takes 40-byte records"
-    c = orc.freestream(0.5, 3.0)
-    ref = oracle_setup(orc, mesh, c)
-    for l in rs5:
+        parts, rec_max, lds = st[0], st[1], st[5]
+        rs = 5 if (rec_max * 48 + 256 * 40 > 40960 or parts < 4 * ncu.value) else 6
+        assert lds == rec_max * 8 * rs + 256 * 40, (l, parts, rec_max, lds)
+        sizes.append(rs)
         n = len(mesh[l]["coords"])
-        var = perturbed_state(n, c.ff_variable[:], seed=31 + l)
+        var = perturbed_state(n, c.ff_variable[:], seed=seed + l)
         for variant in ("part_atomic", "part_coloured"):
             rt.upload(s.lv[l].variables, var)
             rt.upload(s.lv[l].fluxes, np.zeros((n, 5)))
@@ -633,9 +634,21 @@ def test_part_kernels_with_40_byte_records(rt, orc, knn_dense):
             o = np.zeros((n, 5))
             orc.compute_flux_edge(np.ascontiguousarray(mesh[l]["edge_to_node"], np.int32), var, ref[l]["ew"], o, c)
             assert rel_err(s.state(l, "fluxes"), o) <= FLUX_FAST_TOL, (l, variant)
+    return sizes
+
+
+def test_part_kernels_with_40_byte_records(rt, orc, knn_dense):
+    assert 5 in check_part_records(rt, orc, knn_dense, 31)
     # and a device solve through the fused stage kernel
-    ref = orc.solve(mesh, 2, 0.5, 3.0)
-    st2, rms, states = solve_gpu(rt, mesh, 2, 0.5, 3.0, "graph", variant="part_atomic")
+    ref = orc.solve(knn_dense, 2, 0.5, 3.0)
+    st2, rms, states = solve_gpu(rt, knn_dense, 2, 0.5, 3.0, "graph", variant="part_atomic")
     assert st2 == 0
-    for l in range(len(mesh)):
+    for l in range(len(knn_dense)):
         assert rel_err(states[l], ref["levels"][l]["variables"]) <= STATE_TOL, f"level {l}"
+
+
+def test_part_kernels_with_48_byte_records(rt, orc):
+    """A 280K-node level has more parts than workgroup slots and keeps the
+    48-byte records (the level-0 configuration of the headline flux)."""
+    mesh = mgcfd.mesh_knn(280000, 868000, 1, seed=9)
+    assert check_part_records(rt, orc, mesh, 41) == [6]
