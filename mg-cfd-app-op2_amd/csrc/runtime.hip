@@ -1258,20 +1258,35 @@ static int launch_part_e(bool coloured, const InvPlan& P, const Real* w, size_t 
     return launch_part_t<Real, T, EPT, kLarge>(coloured, P, w, lds, var, flux, status, fresh, st);
 }
 
-// Real = double: the FP64 part kernel, 256 threads per part; Real = float:
-// mixed precision, whose smaller register footprint lets 512 threads (8
-// waves) share one part, which shortens each workgroup's edge phase.
-template <typename Real>
-static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
-                       const int* status, int fresh, const PartStage* st = nullptr)
+// Edges per thread from the plan's largest part.
+template <typename Real, int T>
+static int launch_part_n(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                         const int* status, int fresh, const PartStage* st)
 {
-    constexpr int T = sizeof(Real) == 4 ? 512 : 256;
     const int ept = (P.pt_max_ne + T - 1) / T;
     if (ept <= 1024 / T) return launch_part_e<Real, T, 1024 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
     if (ept <= 1536 / T) return launch_part_e<Real, T, 1536 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
     if (ept <= 2048 / T) return launch_part_e<Real, T, 2048 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
     if (ept <= 3072 / T) return launch_part_e<Real, T, 3072 / T>(coloured, P, w, lds, var, flux, status, fresh, st);
     return fail("edge-flux part plan: %d local edges in one part exceed the kernel's %d", P.pt_max_ne, 3072);
+}
+
+// Real = double: the FP64 part kernel, 256 threads per part.  Real = float:
+// mixed precision, whose smaller register footprint lets 512 threads (8
+// waves) share one part in the edge-flux kernel, which shortens each
+// workgroup's edge phase; the kernel with the fused stage needs ~96 VGPRs
+// (its epilogue), which leaves 2 such workgroups per CU, so it runs 256
+// threads per part (5 per CU): batched solves 310 -> 323 per second.
+template <typename Real>
+static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
+                       const int* status, int fresh, const PartStage* st = nullptr)
+{
+    if constexpr (sizeof(Real) == 4) {
+        if (st) return launch_part_n<Real, 256>(coloured, P, w, lds, var, flux, status, fresh, st);
+        return launch_part_n<Real, 512>(coloured, P, w, lds, var, flux, status, fresh, st);
+    } else {
+        return launch_part_n<Real, 256>(coloured, P, w, lds, var, flux, status, fresh, st);
+    }
 }
 
 // Plans and packed weights a flux launch needs; allocates and uploads, so it
