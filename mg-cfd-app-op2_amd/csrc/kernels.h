@@ -650,7 +650,10 @@ __device__ __forceinline__ void part_edges_atomic(const R* rec, double* acc, con
 // Stages one part and accumulates its local edges: returns the FP64
 // accumulators (in LDS, after a barrier) holding the edge flux of the part's
 // owned nodes, started from the flux dat or, with fresh, from zero.
-template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
+// LAZY (coloured only): form each edge's increment in its colour's phase
+// instead of all of them up front, so no increments are held across the
+// phases (fewer registers, for the kernel with the fused stage).
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS, bool LAZY>
 __device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int* __restrict__ halo,
                                                    const unsigned* __restrict__ ecode, const R* __restrict__ ew,
                                                    long long nloc, int rec_cap, const double* __restrict__ var,
@@ -676,7 +679,20 @@ __device__ __forceinline__ double* part_accumulate(const PartInfo& P, const int*
     __syncthreads();
     // 2./3. edges
     const R sc = (R)k.sc;
-    if constexpr (COLOURED) {
+    if constexpr (COLOURED && LAZY) {
+        for (int c = 0; c < P.ncol; c++) {
+#pragma unroll
+            for (int i = 0; i < EPT; i++)
+                if (code[i] && (int)(code[i] >> 22) == c) {
+                    const PState<R> a = part_rec_state<R, RS>(rec + (code[i] & 1023u) * RS);
+                    const PState<R> b = part_rec_state<R, RS>(rec + ((code[i] >> 10) & 1023u) * RS);
+                    R inc[5];
+                    part_increment<R, R>(a, b, w0[i], w1[i], w2[i], sc, inc);
+                    part_apply(acc, code[i], inc);
+                }
+            __syncthreads();
+        }
+    } else if constexpr (COLOURED) {
         R inc[EPT][5];      // held in the arithmetic's precision until the colour walk
 #pragma unroll
         for (int i = 0; i < EPT; i++)
@@ -714,7 +730,8 @@ __global__ __launch_bounds__(T) void k_flux_part(
     if (halted(status)) return;
     const PartInfo P = parts[xcd_block()];
     const double* acc =
-        part_accumulate<R, T, EPT, NPT, COLOURED, RS>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh, sm);
+        part_accumulate<R, T, EPT, NPT, COLOURED, RS, false>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh,
+                                                             sm);
     // 4. store
     double* fw = flux + (size_t)P.n0 * 5;
     for (int j = threadIdx.x; j < P.cnt * 5; j += T) fw[j] = acc[j];
@@ -1355,8 +1372,8 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
     load5(a.old, i, o);
     const double vs = a.rk == 0 ? a.vol[i] : a.sf[i];
     const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
-    double* acc = part_accumulate<R, T, EPT, NPT, COLOURED, RS>(P, halo, ecode, ew, nloc, rec_cap, vin, a.flux,
-                                                                k0, fresh, sm);
+    double* acc = part_accumulate<R, T, EPT, NPT, COLOURED, RS, true>(P, halo, ecode, ew, nloc, rec_cap, vin,
+                                                                      a.flux, k0, fresh, sm);
     double s = 0.0;
     int nb = 0;
     if (t < P.cnt) {
