@@ -41,6 +41,7 @@ import json
 import os
 import platform
 import re
+import subprocess
 import sys
 import time
 
@@ -52,6 +53,8 @@ sys.path.insert(0, os.path.join(ROOT, "mg-cfd-app-op2_amd"))
 import mgcfd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6         # MI355X FP64 vector peak
+FLOP_PER_EDGE = 201             # flux.h:48-207 as written: 186 add/mul + 8 div + 7 sqrt
 N_NODES, N_EDGES = 300_000, 930_000
 MACH, AOA = 0.5, 3.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "flux_traffic.json")
@@ -153,7 +156,7 @@ def bench_solve(rt, cycles=50):
     mesh = mgcfd.mesh_knn(N_NODES, N_EDGES, 4, seed=2)
     out = dict(level_nodes=[len(l["coords"]) for l in mesh], cycles=cycles)
     ref = None
-    for variant in ("gather_exact", "part_atomic", "mixed_atomic"):
+    for variant in ("gather_exact", "part_atomic", "part_coloured", "mixed_atomic"):
         b = mgcfd.Batch(rt, [mesh], [MACH], [AOA], variant)
         b.run(1)
         t0 = time.perf_counter()
@@ -171,18 +174,58 @@ def bench_solve(rt, cycles=50):
     return out, mesh
 
 
-def cpu_solve_cycle(orc, mesh):
-    """One multigrid cycle of the same 4-level mesh in the C oracle (the
-    reference's loops restated, one thread): a 1-cycle solve minus a
-    0-cycle one (the setup loops)."""
+def host_cpu():
+    """The host's CPU model and physical core count (lscpu), the CPUs this
+    process may run on, and the thread count the CPU baselines use: all
+    physical cores, within the affinity mask and OMP_NUM_THREADS."""
+    model, sockets, per_socket = platform.processor() or platform.machine(), 1, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                model = v
+            elif k == "Socket(s)":
+                sockets = int(v)
+            elif k == "Core(s) per socket":
+                per_socket = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    physical = sockets * per_socket if per_socket else (os.cpu_count() or 1)
+    allowed = len(os.sched_getaffinity(0))
+    limit = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or physical
+    return dict(model=model, physical_cores=physical, affinity_cpus=allowed,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"), omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                threads=max(1, min(physical, allowed, limit)))
+
+
+def cpu_solve(mesh, host, cycles=10):
+    """The whole 4-level solve of the same mesh on the host with the
+    reference's kernels (oracle/_ref): 'cycles' cycles with every loop
+    spread over the baseline's threads as OP2's OpenMP backend runs them (on
+    the mesh renumbered as op_renumber would), and one cycle in OP2
+    sequential order on one thread (a 1-cycle solve minus a 0-cycle one)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    if not Oracle.available("ref"):
+        return None
+    ref = Oracle("ref")
+    omp = ref.solve_omp(mesh, cycles, MACH, AOA, threads=host["threads"])
     t = time.perf_counter()
-    orc.solve(mesh, 0, MACH, AOA)
+    ref.solve(mesh, 0, MACH, AOA)
     t0 = time.perf_counter() - t
     t = time.perf_counter()
-    orc.solve(mesh, 1, MACH, AOA)
+    one = ref.solve(mesh, 1, MACH, AOA)
     t1 = time.perf_counter() - t
-    return dict(ms_per_cycle=round((t1 - t0) * 1e3, 1), kind="port", cores=1,
-                sample="one cycle of the 300K-node 4-level solve in the C oracle (setup subtracted), one thread")
+    # the threaded run against the sequential one's first cycle, as a sanity check of the threading
+    return dict(ms_per_cycle=round(omp["seconds"] / cycles * 1e3, 2), cycles=cycles, cores=host["threads"],
+                status=omp["status"], single_thread_ms_per_cycle=round((t1 - t0) * 1e3, 1), kind="reference",
+                sample=f"reference kernels (oracle/_ref): {cycles} cycles of the {len(mesh[0]['coords'])}-node "
+                       f"4-level solve on {host['threads']} threads (OpenMP, edge blocks coloured as OP2's OpenMP "
+                       f"backend colours them, mesh renumbered as op_renumber would; {omp['seconds']:.2f} s) and "
+                       f"one cycle in OP2 sequential order on one thread ({t1 - t0:.2f} s); {host['model']}",
+                rms_rel_diff_vs_sequential=float(f"{abs(omp['rms'][0] / one['rms'][0] - 1):.3e}"))
 
 
 def bench_large(rt, cycles=50, n=8_000_000, mach=0.7, aoa=2.0):
@@ -231,11 +274,11 @@ def check_variant(rt, solver, mesh0, variant, orc):
     return exact, rel
 
 
-def cpu_baseline(mesh0, solver_state, budget_s):
+def cpu_baseline(mesh0, solver_state, budget_s, host):
     """The reference's flux kernel on the host: OpenMP over greedily coloured
-    edges on all the box's cores for this GPU (OMP_NUM_THREADS, else 16), and
-    in OP2 sequential order on one core; each timed on a bounded number of
-    passes over the same mesh (about budget_s / 2 seconds each)."""
+    edges on the baseline's threads (host_cpu), and in OP2 sequential order
+    on one core; each timed on a bounded number of passes over the same mesh
+    (about budget_s / 2 seconds each)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     which = "ref" if Oracle.available("ref") else "orc"
@@ -257,11 +300,11 @@ def cpu_baseline(mesh0, solver_state, budget_s):
     f1 = np.zeros((n, 5))
     calls1, el1 = timed(lambda: o.compute_flux_edge(e2n, var, w, f1, c))
     one = len(e2n) * calls1 / el1 / 1e9
-    proc = platform.processor() or platform.machine()
     if which != "ref":
-        return dict(value=round(one, 5), unit="GEdges/s", cores=1, kind="port",
-                    sample=f"{calls1} sequential passes over the {len(e2n)}-edge level-0 mesh ({el1:.1f} s, {proc})")
-    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+        return dict(value=round(one, 5), unit="GEdges/s", cores=1, kind="port", host=host,
+                    sample=f"{calls1} sequential passes over the {len(e2n)}-edge level-0 mesh ({el1:.1f} s, "
+                           f"{host['model']})")
+    threads = host["threads"]
     col = o.colour_edges(e2n, n)
     fm = np.zeros((n, 5))
     callsm, elm = timed(lambda: o.compute_flux_edge_omp(col, e2n, var, w, fm, c, threads))
@@ -274,10 +317,10 @@ def cpu_baseline(mesh0, solver_state, budget_s):
         raise RuntimeError(f"cpu baseline: coloured OpenMP pass differs from the sequential one by {rel:.2e}")
     many = len(e2n) * callsm / elm / 1e9
     return dict(value=round(many, 5), unit="GEdges/s", cores=threads, kind="reference",
-                single_thread_value=round(one, 5),
+                single_thread_value=round(one, 5), host=host,
                 sample=f"reference flux.h kernel (oracle/_ref): {callsm} OpenMP passes over {len(col[1]) - 1} "
                        f"edge colours on {threads} threads ({elm:.1f} s) and {calls1} sequential passes on one "
-                       f"thread ({el1:.1f} s), {len(e2n)}-edge level-0 mesh, {proc}")
+                       f"thread ({el1:.1f} s), {len(e2n)}-edge level-0 mesh, {host['model']}")
 
 
 def pmc_means(path):
@@ -318,12 +361,16 @@ def summarize_pmc(fetch_csv, write_csv):
     print(json.dumps(out, indent=1))
 
 
-def bench_batched(rt, n_meshes, cycles, seed=40):
+def bench_batched(rt, n_meshes, cycles, fp64_variant, seed=40):
     """64 independent solves (N log-uniform in [20K, 500K], E = 3.1 N, 4
-    levels, Mach U(0.3, 0.8), angle U(-2, 6) deg) batched in mixed precision
-    with the autotuned part size, against the same solves run back to back in
-    plain FP64 (GATHER_EXACT, one captured graph per solve).  Setup is
-    outside both timings."""
+    levels, Mach U(0.3, 0.8), angle U(-2, 6) deg), 30 cycles each:
+      - back to back in plain FP64 with 'fp64_variant' (the fastest FP64
+        flux within 1e-12 of the oracle), one captured graph per solve;
+      - batched in the same FP64 variant (the gain from batching alone);
+      - batched in mixed precision (MIXED_COLOURED) with the autotuned
+        part size (batching and precision together).
+    Setup is outside every timing; each form's first cycle (graph capture) is
+    an untimed warm-up."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(20_000), np.log(500_000), n_meshes)).astype(int)
     mach = rng.uniform(0.3, 0.8, n_meshes)
@@ -332,12 +379,10 @@ def bench_batched(rt, n_meshes, cycles, seed=40):
     meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=1000 + i) for i, n in enumerate(sizes)]
     log(f"bench: batched: generated {n_meshes} meshes, {int(sizes.sum())} level-0 nodes, "
         f"{time.perf_counter() - t:.1f} s")
-    # back-to-back plain FP64 (each solve: one warm-up cycle that captures its
-    # graph, then the timed cycles)
     t_fp64 = 0.0
     ref0 = []
     for i, m in enumerate(meshes):
-        b = mgcfd.Batch(rt, [m], [mach[i]], [aoa[i]], "gather_exact")
+        b = mgcfd.Batch(rt, [m], [mach[i]], [aoa[i]], fp64_variant)
         b.run(1)
         t0 = time.perf_counter()
         st, _ = b.run(cycles)
@@ -346,35 +391,53 @@ def bench_batched(rt, n_meshes, cycles, seed=40):
         b.close()
         if (i + 1) % 16 == 0:
             log(f"bench: batched: {i + 1}/{n_meshes} back-to-back FP64 solves timed ({t_fp64:.2f} s)")
-    # batched mixed precision, same warm-up
-    t = time.perf_counter()
-    b = mgcfd.Batch(rt, meshes, mach, aoa, "mixed_coloured")
-    part = b.autotune(5)
-    t_setup = time.perf_counter() - t
-    b.run(1)
-    t0 = time.perf_counter()
-    st, rms = b.run(cycles)
-    t_batch = time.perf_counter() - t0
-    errs = [float((np.abs(b.fetch(i, 0) - ref0[i]).max(axis=0) / np.abs(ref0[i]).max(axis=0)).max())
-            for i in range(n_meshes)]
-    b.close()
+
+    def batched(variant, tune):
+        t = time.perf_counter()
+        b = mgcfd.Batch(rt, meshes, mach, aoa, variant)
+        part = b.autotune(5) if tune else None
+        t_setup = time.perf_counter() - t
+        b.run(1)
+        t0 = time.perf_counter()
+        st, _ = b.run(cycles)
+        el = time.perf_counter() - t0
+        errs = [float((np.abs(b.fetch(i, 0) - ref0[i]).max(axis=0) / np.abs(ref0[i]).max(axis=0)).max())
+                for i in range(n_meshes)]
+        b.close()
+        out = dict(variant=variant, solves_per_s=round(n_meshes / el, 3), seconds=round(el, 4),
+                   max_rel_err_vs_fp64_alone=float(f"{max(errs):.3e}"), status_ok=bool((st == 0).all()),
+                   setup_seconds=round(t_setup, 2))
+        if part is not None:
+            out["autotuned"] = part
+        return out, el
+
+    fp64b, t_fp64b = batched(fp64_variant, False)
+    log(f"bench: batched: FP64 {fp64b}")
+    mixed, t_mixed = batched("mixed_coloured", True)
+    log(f"bench: batched: mixed {mixed}")
     out = dict(n_solves=n_meshes, cycles=cycles, level0_nodes=int(sizes.sum()),
-               solves_per_s=round(n_meshes / t_batch, 3), seconds=round(t_batch, 4),
-               fp64_back_to_back_solves_per_s=round(n_meshes / t_fp64, 3),
-               fp64_back_to_back_seconds=round(t_fp64, 4), speedup=round(t_fp64 / t_batch, 3),
-               max_rel_err_vs_fp64=float(f"{max(errs):.3e}"), autotuned_part_nodes=part,
-               status_ok=bool((st == 0).all()), setup_seconds=round(t_setup, 2),
-               variant="mixed_coloured (FP32 flux, FP64 accumulation)")
+               solves_per_s=mixed["solves_per_s"], seconds=mixed["seconds"],
+               variant="mixed_coloured (FP32 flux, FP64 accumulation)",
+               fp64_back_to_back=dict(variant=fp64_variant, solves_per_s=round(n_meshes / t_fp64, 3),
+                                      seconds=round(t_fp64, 4)),
+               fp64_batched=fp64b, mixed_batched=mixed,
+               speedup=round(t_fp64 / t_mixed, 3), speedup_from_batching=round(t_fp64 / t_fp64b, 3),
+               speedup_from_precision=round(t_fp64b / t_mixed, 3),
+               max_rel_err_vs_fp64=mixed["max_rel_err_vs_fp64_alone"],
+               status_ok=fp64b["status_ok"] and mixed["status_ok"])
     log(f"bench: batched: {out}")
     return out
 
 
-def bench_serving(rt, n_req, rate, seed=50):
-    """Poisson arrivals at 'rate' requests/s; N lognormal clipped to [10K, 2M]
-    nodes (4 levels); Mach/angle as batched; FP64 (PART_COLOURED) / mixed
-    precision (MIXED_COLOURED) 50/50;
-    each runs to an L0 RMS of 1e-6 x its first cycle's or 200 cycles; 200 GB
-    admission budget.  Latency is submit -> result on the host."""
+def bench_serving(rt, n_req, rates, seed=50):
+    """Poisson arrivals of n_req requests at each rate of 'rates' (requests/s;
+    0 = all at once); N lognormal clipped to [10K, 2M] nodes (4 levels);
+    Mach/angle as batched; FP64 (PART_COLOURED) / mixed precision
+    (MIXED_COLOURED) 50/50; each runs to an L0 RMS of 1e-6 x its first
+    cycle's or 200 cycles; 200 GB admission budget, at most 8 requests on the
+    device at once.  Latency is submit -> result on the host.  The rates run
+    from below to past the server's capacity: the last one measures the
+    saturated throughput."""
     rng = np.random.default_rng(seed)
     sizes = np.clip(np.exp(rng.normal(np.log(100_000), 1.0, n_req)), 10_000, 2_000_000).astype(int)
     mach = rng.uniform(0.3, 0.8, n_req)
@@ -382,44 +445,58 @@ def bench_serving(rt, n_req, rate, seed=50):
     # FP64 requests run the deterministic FP64 part kernels (within 1e-12 of
     # the exact form, and each request's result independent of the others)
     variant = np.where(rng.uniform(size=n_req) < 0.5, "part_coloured", "mixed_coloured")
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    gaps = rng.exponential(1.0, n_req)
     t = time.perf_counter()
     meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=2000 + i) for i, n in enumerate(sizes)]
     log(f"bench: serving: generated {n_req} meshes ({int(sizes.min())}..{int(sizes.max())} nodes), "
         f"{time.perf_counter() - t:.1f} s")
-    # every request's 200 cycles are issued at admission (one stream of graph
-    # replays); the convergence stop halts the rest on the device
-    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=200)
-    ids, done = [], {}
-    t0 = time.perf_counter()
-    k = 0
-    while len(done) < n_req:
-        now = time.perf_counter() - t0
-        while k < n_req and arrivals[k] <= now:
-            ids.append(srv.submit(meshes[k], float(mach[k]), float(aoa[k]), str(variant[k]), max_cycles=200,
-                                  rms_drop=1e-6))
-            k += 1
-        srv.progress()
-        for i in ids:
-            if i not in done:
-                r = srv.result(i)
-                if r is not None:
-                    done[i] = r
-        if now > 600:
-            srv.close()
-            raise RuntimeError(f"serving: only {len(done)} of {n_req} requests finished in 600 s")
-        time.sleep(0.0005)
-    wall = time.perf_counter() - t0
-    srv.close()
-    lat = np.array([done[i]["latency"] for i in ids])
-    cyc = np.array([done[i]["cycles"] for i in ids])
-    out = dict(requests=n_req, arrival_rate_per_s=rate, wall_seconds=round(wall, 3),
-               solves_per_s=round(n_req / wall, 3), p50_latency_s=round(float(np.percentile(lat, 50)), 4),
-               p99_latency_s=round(float(np.percentile(lat, 99)), 4), mean_cycles=float(cyc.mean()),
-               nodes_min=int(sizes.min()), nodes_max=int(sizes.max()),
-               status_ok=all(done[i]["status"] in (0, 3) for i in ids))
-    log(f"bench: serving: {out}")
-    return out
+    sweep = []
+    for rate in rates:
+        arrivals = np.cumsum(gaps / rate) if rate > 0 else np.zeros(n_req)
+        # every request's 200 cycles are issued at admission (one stream of
+        # graph replays); the convergence stop halts the rest on the device
+        srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=200)
+        ids, done = [], {}
+        t0 = time.perf_counter()
+        k = 0
+        while len(done) < n_req:
+            now = time.perf_counter() - t0
+            while k < n_req and arrivals[k] <= now:
+                ids.append(srv.submit(meshes[k], float(mach[k]), float(aoa[k]), str(variant[k]), max_cycles=200,
+                                      rms_drop=1e-6))
+                k += 1
+            srv.progress()
+            for i in ids:
+                if i not in done:
+                    r = srv.result(i)
+                    if r is not None:
+                        done[i] = r
+            if now > 600:
+                srv.close()
+                raise RuntimeError(f"serving: only {len(done)} of {n_req} requests finished in 600 s")
+            time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        spans = [srv.timing(i) for i in ids]
+        srv.close()
+        lat = np.array([done[i]["latency"] for i in ids])
+        cyc = np.array([done[i]["cycles"] for i in ids])
+        # the most requests whose device intervals overlap at one instant
+        ev = sorted([(a, 1) for a, _ in spans] + [(b, -1) for _, b in spans], key=lambda x: (x[0], x[1]))
+        cur = peak = 0
+        for _, d in ev:
+            cur += d
+            peak = max(peak, cur)
+        out = dict(arrival_rate_per_s=rate if rate > 0 else "all at once", wall_seconds=round(wall, 3),
+                   solves_per_s=round(n_req / wall, 3), p50_latency_s=round(float(np.percentile(lat, 50)), 4),
+                   p99_latency_s=round(float(np.percentile(lat, 99)), 4), mean_cycles=float(cyc.mean()),
+                   max_requests_on_device=peak, status_ok=all(done[i]["status"] in (0, 3) for i in ids))
+        log(f"bench: serving: {out}")
+        sweep.append(out)
+    sat = sweep[-1]
+    return dict(requests=n_req, nodes_min=int(sizes.min()), nodes_max=int(sizes.max()),
+                solves_per_s=sat["solves_per_s"], p50_latency_s=sat["p50_latency_s"],
+                p99_latency_s=sat["p99_latency_s"], saturated_at=sat["arrival_rate_per_s"], sweep=sweep,
+                status_ok=all(x["status_ok"] for x in sweep))
 
 
 def main():
@@ -435,7 +512,8 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=64)
     ap.add_argument("--batch-cycles", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rates", default="4,16,64,0",
+                    help="Poisson arrival rates swept, requests/s (0: all at once)")
     ap.add_argument("--no-shard", action="store_true", help="skip the 1M-node shard-shape flux measurement")
     ap.add_argument("--no-solve", action="store_true", help="skip the 300K-node 50-cycle solve timing")
     ap.add_argument("--large", action="store_true", help="also run the 8M-node, 50-cycle solve (about 2 minutes)")
@@ -450,6 +528,12 @@ def main():
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baselines' OpenMP threads stay on their cores, as the
+    # reference's run scripts ask (run-templates/local.sh:3-4); set before the
+    # reference-kernel library (and its OpenMP runtime) loads
+    os.environ.setdefault("OMP_PROC_BIND", "true")
+    host = host_cpu()
+    log(f"bench: host CPU {host}")
     log("bench: generating the 300K-node / 930K-edge mesh")
     mesh0 = mgcfd.mesh_knn(N_NODES, N_EDGES, 1, seed=2)[0]
     nb = algorithmic_bytes(N_NODES, N_EDGES)
@@ -509,6 +593,9 @@ def main():
     log(f"bench: unstructured stream: {s_rot * 1e3:.2f} us/call rotating, {s_res * 1e3:.2f} us/call resident; "
         f"flux/stream {stream['flux_over_stream']} ({best})")
     achieved = nb / (ms * 1e-3) / 1e9
+    tflops = FLOP_PER_EDGE * N_EDGES / (ms * 1e-3) / 1e12
+    log(f"bench: {best}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of HBM peak), "
+        f"{tflops:.2f} FP64 TFLOP/s ({tflops / FP64_PEAK_TFLOPS:.3f} of the FP64 vector peak)")
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
@@ -518,17 +605,26 @@ def main():
     if not args.no_cpu:
         var = rt.fetch(solvers[0].lv[0].variables)
         w = rt.fetch(solvers[0].lv[0].edge_weights)
-        cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds)
+        cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds, host)
     shard = None if args.no_shard else bench_shard(rt, best, args.steps, args.warmup)
     solve = None
     if not args.no_solve:
         solve, solve_mesh = bench_solve(rt)
         if cpu is not None:
-            cpu["solve"] = cpu_solve_cycle(orc, solve_mesh)
+            cpu["solve"] = cpu_solve(solve_mesh, host)
             log(f"bench: cpu solve: {cpu['solve']}")
         del solve_mesh
-    batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles)
-    serving = None if args.no_serve else bench_serving(rt, args.serve_requests, args.serve_rate)
+    # the back-to-back FP64 baseline runs the fastest FP64 flux within 1e-12
+    # of the oracle in the whole-solve measurement (part_atomic unless the
+    # solve leg was skipped)
+    fp64_solver = "part_atomic"
+    if solve is not None:
+        cand = [v for v in ("gather_exact", "part_atomic", "part_coloured") if v in solve
+                and (v == "gather_exact" or solve[v].get("max_rel_err_vs_gather_exact", 1.0) <= 1e-12)]
+        fp64_solver = min(cand, key=lambda v: solve[v]["ms_per_cycle"])
+    batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles, fp64_solver)
+    serving = None if args.no_serve else bench_serving(rt, args.serve_requests,
+                                                           [float(x) for x in args.serve_rates.split(",")])
     large = bench_large(rt) if args.large else None
     rt.close()
 
@@ -551,7 +647,12 @@ def main():
                    "working_set_MB": round(args.copies * nb / 1e6, 1),
                    "algorithmic_bytes_per_call": nb, "copy_kernel_GBs": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/flux_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this "
+                                       "script, scaled by what they read for k_stream_copy of 512 MiB (not measured "
+                                       "in this run)" if traffic else None,
+                     "fp64_tflops": round(tflops, 3), "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                     "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_edge": FLOP_PER_EDGE},
         "per_call": dist,
         "cpu_baseline": cpu,
         "stream": stream,

@@ -370,6 +370,15 @@ int mgcfd_server_progress(mgcfd_server s, int* n_active, int* n_queued);
  * that could not be set up returns an error. */
 int mgcfd_server_result(mgcfd_server s, int request_id, int* state, int* status, int* cycles, double* latency_s,
                         double* variables, double* rms);
+/* Device timeline of a finished request: when its stream reached its first
+ * cycle and when its last cycle completed, in ms since the server was
+ * created (HIP events on the request's stream).  Two requests overlapped on
+ * the device when their intervals intersect. */
+int mgcfd_server_timing(mgcfd_server s, int request_id, double* device_start_ms, double* device_end_ms);
+/* The device-memory estimate that admission charges for a mesh.
+ * mgcfd_server_submit refuses a request whose estimate exceeds the whole
+ * budget. */
+int mgcfd_server_estimate_bytes(const mgcfd_mesh* mesh, double* bytes);
 int mgcfd_server_destroy(mgcfd_server s);
 
 /* Times 'iters' launches of a plain device-to-device copy of 'bytes' bytes

@@ -1032,6 +1032,46 @@ __device__ __forceinline__ int block_sum_i(int x)
     return x;
 }
 
+// Publication of a level's minimum dt by the kernels that compute dt
+// (get_min_dt_kernel, time_stepping_kernels.h:34-41, fused into them): every
+// workgroup writes its minimum to part[blockIdx.x] and counts itself in its
+// mesh's arrival counter; the mesh's last workgroup to arrive reduces the
+// mesh's minima, writes the result to out[seg] and resets the counter, so
+// the counters are zero again for the next launch (also in a replayed
+// graph).  min is exact: the order of arrival does not change the result.
+struct MinPub {
+    const int* sblk;    // first block of each segment (null: one segment of gridDim.x blocks)
+    unsigned* cnt;      // arrival counter per segment, zero between launches
+    double* out;        // minimum per segment
+};
+
+// m: the workgroup's minimum (valid in thread 0).
+template <int NT = kBlock>
+__device__ __forceinline__ void min_publish(const MinPub& p, double* __restrict__ part, int seg, double m)
+{
+    __shared__ int last;
+    const int lo = p.sblk ? p.sblk[seg] : 0, hi = p.sblk ? p.sblk[seg + 1] : (int)gridDim.x;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = m;
+        __threadfence();
+        last = atomicAdd(p.cnt + seg, 1u) == (unsigned)(hi - lo - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double x = DBL_MAX;
+    for (int j = lo + (int)threadIdx.x; j < hi; j += NT) {
+        const double y = __longlong_as_double((long long)__hip_atomic_load(
+            reinterpret_cast<unsigned long long*>(part + j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        x = (y < x) ? y : x;
+    }
+    x = block_min<NT>(x);
+    if (threadIdx.x == 0) {
+        p.out[seg] = x;
+        __hip_atomic_store(p.cnt + seg, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Candidates for get_min_dt_kernel: like the reference's 'dt < min_dt' test
 // starting from DBL_MAX, NaN and values >= DBL_MAX never win.
 __device__ __forceinline__ double min_candidate(double d) { return (d < DBL_MAX) ? d : DBL_MAX; }
@@ -1077,22 +1117,20 @@ struct DtArgs {
     double* dt;
     double* old;
     double* part;
+    MinPub pub;
 };
 
 __global__ __launch_bounds__(kBlock) void k_calc_dt(
     int n, const double* __restrict__ var, const double* __restrict__ vol,
     double* __restrict__ dt, double* __restrict__ old, double* __restrict__ part,
-    const int* __restrict__ status)
+    const int* __restrict__ status, MinPub pub)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = DBL_MAX;
     if (i < n) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
-    if (part) {
-        const double m = block_min(d);
-        if (threadIdx.x == 0) part[blockIdx.x] = m;
-    }
+    if (part) min_publish(pub, part, 0, block_min(d));
 }
 
 // get_min_dt_kernel as a standalone loop: per-workgroup min of dt.
@@ -1166,10 +1204,8 @@ __global__ void k_time_step(int n, int rk, const double* __restrict__ sf, double
 struct StageArgs {
     int n, rk;
     const int4* blk;           // batched solves: {segment, first, end} per workgroup; null: plain blocks over n
-    const double* mpart;       // first stage: per-workgroup minima of k_calc_dt ...
-    int np;                    // ... [0, np) for one mesh, or
-    const int* sblk;           // ... [sblk[s], sblk[s + 1]) for segment s
-    double* mout;              // minimum dt written: [0], or per segment
+    const int* sblk;           // batched solves: first workgroup of each segment
+    const double* mdt;         // first stage: minimum dt published by the dt kernel: [0], or per segment
     int* mstatus;              // abort code 1 on a negative minimum: one status, or per segment
     int* zero;                 // first stage: a counter the last stage accumulates into, cleared
     const double* vol;
@@ -1188,20 +1224,14 @@ struct StageArgs {
     const int* status;
 };
 
-// First stage: the minimum dt of the workgroup's mesh, from the k_calc_dt
-// per-workgroup minima (min is exact, so every workgroup gets the same
-// value); the mesh's first workgroup publishes it and raises abort code 1 on
-// a negative value (euler3d.cpp:476-484).
+// First stage: the minimum dt of the workgroup's mesh, as the kernel that
+// computed dt published it (min_publish); the mesh's first workgroup raises
+// abort code 1 on a negative value (euler3d.cpp:476-484).
 template <int NT>
 __device__ __forceinline__ double stage_min_dt(const StageArgs& a, int seg, bool first)
 {
-    const int lo = a.blk ? a.sblk[seg] : 0, hi = a.blk ? a.sblk[seg + 1] : a.np;
-    double m = DBL_MAX;
-    for (int j = lo + (int)threadIdx.x; j < hi; j += NT) m = (a.mpart[j] < m) ? a.mpart[j] : m;
-    m = block_min_all<NT>(m);
-    const double mdt = (m < DBL_MAX) ? m : DBL_MAX;
+    const double mdt = a.mdt[seg];
     if (first && threadIdx.x == 0) {
-        a.mout[seg] = mdt;
         if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
         if (a.zero) *a.zero = 0;
     }
@@ -1640,8 +1670,7 @@ __global__ __launch_bounds__(kBlock) void k_up_gather_dt(
         }
         m = min_candidate(calc_dt_state(c, acc, d.vol, d.dt, d.old));
     }
-    m = block_min(m);
-    if (threadIdx.x == 0) d.part[blockIdx.x] = m;
+    min_publish(d.pub, d.part, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
 }
 
 __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__ vf,
@@ -1783,8 +1812,7 @@ __global__ __launch_bounds__(kBlock) void k_down_dt(
         store5(vf, i, u);
         m = min_candidate(calc_dt_state(i, u, d.vol, d.dt, d.old));
     }
-    m = block_min(m);
-    if (threadIdx.x == 0) d.part[blockIdx.x] = m;
+    min_publish(d.pub, d.part, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
 }
 
 // ------------------------------------------------------------------------
@@ -1845,15 +1873,14 @@ __global__ void k_cell_volumes_node(int n_targets, const int* __restrict__ targe
 __global__ __launch_bounds__(kBlock) void k_calc_dt_seg(const int4* __restrict__ blk, const double* __restrict__ var,
                                                         const double* __restrict__ vol, double* __restrict__ dt,
                                                         double* __restrict__ old, double* __restrict__ part,
-                                                        const int* __restrict__ halt)
+                                                        const int* __restrict__ halt, MinPub pub)
 {
     if (halted(halt)) return;
     const int4 B = blk[blockIdx.x];
     const int i = B.y + threadIdx.x;
     double d = DBL_MAX;
     if (i < B.z) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
-    const double m = block_min(d);
-    if (threadIdx.x == 0) part[blockIdx.x] = m;
+    min_publish(pub, part, B.x, block_min(d));
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,

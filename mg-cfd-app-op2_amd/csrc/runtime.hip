@@ -1856,7 +1856,7 @@ int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_
     TRY(direct(a2, set, 1, false, L, "arg2 dt[1] OP_WRITE"));
     if (set->size)
         k_calc_dt<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
-                                                            nullptr, nullptr, nullptr);
+                                                            nullptr, nullptr, nullptr, MinPub{});
     touch(a2.dat);
     return check_launch();
 }

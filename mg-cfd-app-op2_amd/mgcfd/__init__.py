@@ -449,6 +449,21 @@ class Server:
             return None
         return dict(status=st.value, cycles=cyc.value, latency=lat.value, variables=var, rms=rms[:cyc.value])
 
+    def timing(self, rid: int):
+        """(device start, device end) of a finished request, ms since the server was created."""
+        a, b = C.c_double(0), C.c_double(0)
+        _check(self.L.mgcfd_server_timing(self.h, rid, C.byref(a), C.byref(b)), "mgcfd_server_timing")
+        return a.value, b.value
+
+    @staticmethod
+    def estimate_bytes(mesh: list[dict]) -> float:
+        """The device-memory estimate admission charges for this mesh."""
+        keep = []
+        m = _mesh_c(mesh, keep)
+        b = C.c_double(0)
+        _check(lib().mgcfd_server_estimate_bytes(C.byref(m), C.byref(b)), "mgcfd_server_estimate_bytes")
+        return b.value
+
     def close(self):
         if self.h:
             _check(self.L.mgcfd_server_destroy(self.h), "mgcfd_server_destroy")

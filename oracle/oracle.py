@@ -169,15 +169,9 @@ class Oracle:
                                          C.byref(c), threads)
 
     # --- whole solve
-    def solve(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, consts: Consts | None = None,
-              after_setup=None, down_v2=False):
-        """Setup (euler3d.cpp:412-441) + multigrid loop (:452-641) on copies of
-        the mesh arrays.  after_setup(list of per-level array dicts) may modify
-        the state between the two (failure-path tests).  down_v2 prolongs with
-        the inverse-distance loops (euler3d.cpp:603-623) over the edge-->mg_nodes
-        map e2mg[e][k] = node_to_mg_node[e2n[e][k]].  Returns
-        dict(status, rms, min_dt, steps, levels=[...])."""
-        c = consts if consts is not None else self.freestream(mach, aoa)
+    @staticmethod
+    def _levels(mesh: list[dict], down_v2: bool = False):
+        """Working copies of every level's arrays and the C level table over them."""
         nl = len(mesh)
         arrs = []
         levels = (Level * nl)()
@@ -207,16 +201,117 @@ class Oracle:
             L.fluxes, L.step_factors, L.volumes, L.up_scratch = _p(a["flux"]), _p(a["sf"]), _p(a["vol"]), _p(a["scr"])
             L.edge_to_mg_nodes = _p(a["e2mg"]) if a["e2mg"] is not None else None
             L.prolonged, L.prolonged_wsum = _p(a["rp"]), _p(a["wsum"])
+        return arrs, levels
+
+    def solve(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, consts: Consts | None = None,
+              after_setup=None, down_v2=False, checkpoints=()):
+        """Setup (euler3d.cpp:412-441) + multigrid loop (:452-641) on copies of
+        the mesh arrays.  after_setup(list of per-level array dicts) may modify
+        the state between the two (failure-path tests).  down_v2 prolongs with
+        the inverse-distance loops (euler3d.cpp:603-623) over the edge-->mg_nodes
+        map e2mg[e][k] = node_to_mg_node[e2n[e][k]].  checkpoints: cycle
+        counts below 'cycles' after which every level's variables are copied
+        (each multigrid cycle starts and ends at level 0, so the run in pieces
+        is the run in one).  Returns dict(status, rms, min_dt, steps,
+        levels=[...], snapshots={cycle: [variables per level]})."""
+        c = consts if consts is not None else self.freestream(mach, aoa)
+        nl = len(mesh)
+        arrs, levels = self._levels(mesh, down_v2)
         self.fn("setup")(levels, nl, C.byref(c))
         if after_setup is not None:
             after_setup(arrs)
         rms = np.zeros(max(cycles, 1))
         mdt = np.zeros(max(1, cycles * max(1, 2 * nl - 2) + 8))
         steps = C.c_int(0)
-        st = self.fn("mg_run")(levels, nl, cycles, C.byref(c), _p(rms), _p(mdt), C.byref(steps))
+        snaps, done, nsteps, st = {}, 0, 0, 0
+        for k in sorted({int(x) for x in checkpoints if 0 < x < cycles}) + [cycles]:
+            st = self.fn("mg_run")(levels, nl, k - done, C.byref(c), _p(rms[done:]), _p(mdt[nsteps:]),
+                                   C.byref(steps))
+            nsteps += steps.value
+            done = k
+            if st != 0:
+                break
+            if k < cycles:
+                snaps[k] = [a["var"].copy() for a in arrs]
+        steps.value = nsteps
         out = []
         for a in arrs:
             out.append(dict(variables=a["var"], old_variables=a["old"], residuals=a["res"], fluxes=a["flux"],
                             step_factors=a["sf"], volumes=a["vol"], edge_weights=a["ew"], bnd_weights=a["bw"],
                             up_scratch=a["scr"], prolonged=a["rp"], prolonged_wsum=a["wsum"]))
-        return dict(status=st, rms=rms[:cycles], min_dt=mdt[:steps.value], steps=steps.value, levels=out)
+        return dict(status=st, rms=rms[:cycles], min_dt=mdt[:steps.value], steps=steps.value, levels=out,
+                    snapshots=snaps)
+
+    @staticmethod
+    def renumber(mesh: list[dict]):
+        """The mesh with every level's nodes in Z-order of their coordinates,
+        edges sorted by their renumbered first node and boundary elements by
+        node -- what op_renumber does for OP2's CPU backends (euler3d.cpp:
+        340-377) so that a block of consecutive edges touches a compact patch
+        of nodes.  Returns (mesh, perms) with perms[l][new] = old."""
+        perms = []
+        for m in mesh:
+            x = np.asarray(m["coords"], dtype=np.float64)
+            lo, hi = x.min(axis=0), x.max(axis=0)
+            q = ((x - lo) / np.maximum(hi - lo, 1e-300) * (2 ** 21 - 1)).astype(np.uint64)
+            key = np.zeros(len(x), dtype=np.uint64)
+            for bit in range(21):
+                for ax in range(3):
+                    key |= ((q[:, ax] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + ax)
+            perms.append(np.argsort(key, kind="stable"))
+        out = []
+        for l, m in enumerate(mesh):
+            p = perms[l]
+            inv = np.empty_like(p)
+            inv[p] = np.arange(len(p))
+            e2n = inv[np.asarray(m["edge_to_node"])]
+            eo = np.argsort(e2n[:, 0], kind="stable")
+            b2n = inv[np.asarray(m["bnd_to_node"]).reshape(-1)]
+            bo = np.argsort(b2n, kind="stable")
+            r = dict(coords=np.asarray(m["coords"])[p], edge_to_node=e2n[eo],
+                     edge_weights=np.asarray(m["edge_weights"])[eo], bnd_to_node=b2n[bo],
+                     bnd_group=np.asarray(m["bnd_group"]).reshape(-1)[bo],
+                     bnd_weights=np.asarray(m["bnd_weights"])[bo])
+            if m.get("node_to_mg_node") is not None and l + 1 < len(mesh):
+                pc = perms[l + 1]
+                invc = np.empty_like(pc)
+                invc[pc] = np.arange(len(pc))
+                r["node_to_mg_node"] = invc[np.asarray(m["node_to_mg_node"]).reshape(-1)[p]]
+            out.append(r)
+        return out, perms
+
+    def solve_omp(self, mesh: list[dict], cycles: int, mach=1.2, aoa=0.0, threads: int = 1):
+        """The reference kernels' whole solve with every loop spread over
+        'threads' host threads as OP2's OpenMP backend runs it, on the mesh
+        renumbered as op_renumber would (ref library only).  A timing
+        baseline: the block colours change the order of each node's flux
+        increments.  Setup, renumbering and the thread plan are outside the
+        timed region.  Returns dict(status, rms, seconds, levels=[variables
+        per level, original node order])."""
+        if self.p != "ref":
+            raise RuntimeError("solve_omp needs the reference-kernel library")
+        import time
+        c = self.freestream(mach, aoa)
+        nl = len(mesh)
+        mesh, perms = self.renumber(mesh)
+        arrs, levels = self._levels(mesh)
+        self.fn("setup")(levels, nl, C.byref(c))
+        self.lib.ref_omp_plan_create.restype = C.c_void_p
+        self.lib.ref_omp_plan_create.argtypes = [C.POINTER(Level), C.c_int]
+        self.lib.ref_omp_plan_free.argtypes = [C.c_void_p]
+        self.lib.ref_mg_run_omp.argtypes = [C.c_void_p, C.POINTER(Level), C.c_int, C.c_int, C.POINTER(Consts), _PD,
+                                            C.c_int]
+        plan = self.lib.ref_omp_plan_create(levels, nl)
+        if not plan:
+            raise RuntimeError("solve_omp: edge-block colouring needs more than 256 colours")
+        rms = np.zeros(max(cycles, 1))
+        t = time.perf_counter()
+        st = self.lib.ref_mg_run_omp(plan, levels, nl, cycles, C.byref(c), _p(rms), threads)
+        el = time.perf_counter() - t
+        self.lib.ref_omp_plan_free(plan)
+        states = []
+        for a, p in zip(arrs, perms):
+            v = np.empty_like(a["var"])
+            v[p] = a["var"]
+            states.append(v)
+        return dict(status=st, rms=rms[:cycles], seconds=el, levels=states)

@@ -11,6 +11,7 @@
 // functions so the tests can compare the two bit for bit.  The globals the
 // reference's euler3d.cpp defines (euler3d.cpp:47-57) are defined here and
 // filled from the caller's constants before each call.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -378,6 +379,237 @@ int ref_mg_run(orc_level* L, int nlev, int cycles, const orc_consts* c, double* 
         }
     }
     if (steps_done) *steps_done = steps;
+    return 0;
+}
+
+// ---- multi-threaded CPU baseline of the whole solve ----
+//
+// The same multigrid loop with every op_par_loop spread over host threads the
+// way OP2's OpenMP backend spreads it: direct loops split over the set; the
+// edge flux one parallel loop per colour of edge blocks (no two blocks of a
+// colour share a node), each block in edge order; the indirect loops onto a
+// node (boundary flux, restriction) one thread per target node, each walking
+// its elements in element order;
+// the min / sum reductions per thread, then combined.  The kernel bodies are
+// the reference's.  Timing baseline only: the block colours change the order
+// of each node's flux increments (ulp-level), and the rms sum order differs.
+struct RefOmpLevel {
+    std::vector<int> cptr, order;                 // block colours: blocks of colour k are order[cptr[k] ..
+                                                  // cptr[k + 1]), block b = edges [b * kEB, (b + 1) * kEB)
+    std::vector<int> bptr, bent;                  // boundary elements per node (CSR, element order)
+    std::vector<int> bnodes;                      // nodes with boundary elements
+    std::vector<int> fptr, fine;                  // fine children per coarse node (into this level's
+                                                  // coarser neighbour), element order
+};
+
+struct RefOmpPlan {
+    std::vector<RefOmpLevel> L;
+};
+
+// OP2's OpenMP backend executes an indirect loop as blocks of consecutive set
+// elements, coloured so that no two blocks of one colour increment the same
+// node; blocks of a colour run in parallel, each block in element order.
+static const int kEB = 256;
+
+// Greedy block colouring of an edge set; returns the colour count, -1 if
+// more than 256 are needed.
+static int colour_edge_blocks(int n_edges, int n_nodes, const int* e2n, std::vector<int>& order,
+                              std::vector<int>& cptr)
+{
+    const int nb = (n_edges + kEB - 1) / kEB;
+    constexpr int W = 4;                      // 4 x 64 colour bits per node
+    std::vector<unsigned long long> used((size_t)n_nodes * W, 0ull);
+    std::vector<int> col((size_t)nb);
+    int nc = 0;
+    for (int b = 0; b < nb; b++) {
+        unsigned long long u[W] = {0, 0, 0, 0};
+        const int e1 = std::min(n_edges, (b + 1) * kEB);
+        for (int e = b * kEB; e < e1; e++)
+            for (int k = 0; k < 2; k++) {
+                const size_t v = (size_t)e2n[2 * (size_t)e + k];
+                for (int w = 0; w < W; w++) u[w] |= used[v * W + w];
+            }
+        int c = -1;
+        for (int w = 0; w < W && c < 0; w++)
+            if (~u[w]) c = 64 * w + __builtin_ctzll(~u[w]);
+        if (c < 0) return -1;
+        col[b] = c;
+        for (int e = b * kEB; e < e1; e++)
+            for (int k = 0; k < 2; k++) {
+                const size_t v = (size_t)e2n[2 * (size_t)e + k];
+                used[v * W + c / 64] |= 1ull << (c % 64);
+            }
+        nc = std::max(nc, c + 1);
+    }
+    cptr.assign((size_t)nc + 1, 0);
+    for (int b = 0; b < nb; b++) cptr[col[b] + 1]++;
+    for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+    order.resize((size_t)nb);
+    std::vector<int> pos(cptr.begin(), cptr.end() - 1);
+    for (int b = 0; b < nb; b++) order[(size_t)pos[col[b]]++] = b;
+    return nc;
+}
+
+static void csr_by_target(int n_src, const int* tgt, int n_tgt, std::vector<int>& ptr, std::vector<int>& ent)
+{
+    ptr.assign((size_t)n_tgt + 1, 0);
+    for (int i = 0; i < n_src; i++) ptr[(size_t)tgt[i] + 1]++;
+    for (int t = 0; t < n_tgt; t++) ptr[t + 1] += ptr[t];
+    ent.resize((size_t)n_src);
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n_src; i++) ent[(size_t)pos[tgt[i]]++] = i;
+}
+
+void* ref_omp_plan_create(const orc_level* L, int nlev)
+{
+    RefOmpPlan* P = new RefOmpPlan();
+    P->L.resize(nlev);
+    for (int l = 0; l < nlev; l++) {
+        RefOmpLevel& q = P->L[l];
+        if (colour_edge_blocks(L[l].n_edges, L[l].n_nodes, L[l].edge_to_node, q.order, q.cptr) < 0) {
+            delete P;
+            return nullptr;
+        }
+        csr_by_target(L[l].n_bnd, L[l].bnd_to_node, L[l].n_nodes, q.bptr, q.bent);
+        for (int i = 0; i < L[l].n_nodes; i++)
+            if (q.bptr[i + 1] > q.bptr[i]) q.bnodes.push_back(i);
+        if (l + 1 < nlev) csr_by_target(L[l].n_nodes, L[l].node_to_mg_node, L[l + 1].n_nodes, q.fptr, q.fine);
+    }
+    return P;
+}
+
+void ref_omp_plan_free(void* plan) { delete static_cast<RefOmpPlan*>(plan); }
+
+static void omp_smooth_level(const RefOmpLevel& q, orc_level* l, int threads, double* min_dt_out)
+{
+    const int n = l->n_nodes;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int i = 0; i < n; i++) copy_double_kernel(l->variables + 5 * (size_t)i, l->old_variables + 5 * (size_t)i);
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int i = 0; i < n; i++)
+        calculate_dt_kernel(l->variables + 5 * (size_t)i, l->volumes + i, l->step_factors + i);
+    double m = std::numeric_limits<double>::max();
+#pragma omp parallel num_threads(threads)
+    {
+        double mt = std::numeric_limits<double>::max();
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; i++) get_min_dt_kernel(l->step_factors + i, &mt);
+#pragma omp critical
+        get_min_dt_kernel(&mt, &m);
+    }
+    *min_dt_out = m;
+}
+
+static void omp_stages(const RefOmpLevel& q, orc_level* l, double min_dt, int threads)
+{
+    const int n = l->n_nodes;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int i = 0; i < n; i++)
+        compute_step_factor_kernel(l->variables + 5 * (size_t)i, l->volumes + i, &min_dt, l->step_factors + i);
+    const int nc = (int)q.cptr.size() - 1;
+    const int nbn = (int)q.bnodes.size();
+    for (int rk = 0; rk < RK; rk++) {
+        for (int k = 0; k < nc; k++) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic)
+            for (int j = q.cptr[k]; j < q.cptr[k + 1]; j++) {
+                const int bk = q.order[j];
+                const int e1 = std::min(l->n_edges, (bk + 1) * kEB);
+                for (int ei = bk * kEB; ei < e1; ei++) {
+                    const size_t e = (size_t)ei;
+                    const size_t a = (size_t)l->edge_to_node[2 * e], b = (size_t)l->edge_to_node[2 * e + 1];
+                    compute_flux_edge_kernel(l->variables + 5 * a, l->variables + 5 * b, l->edge_weights + 3 * e,
+                                             l->fluxes + 5 * a, l->fluxes + 5 * b);
+                }
+            }
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int t = 0; t < nbn; t++) {
+            const size_t v = (size_t)q.bnodes[t];
+            for (int j = q.bptr[v]; j < q.bptr[v + 1]; j++) {
+                const size_t bi = (size_t)q.bent[j];
+                compute_bnd_node_flux_kernel(l->bnd_group + bi, l->bnd_weights + 3 * bi, l->variables + 5 * v,
+                                             l->fluxes + 5 * v);
+            }
+        }
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int i = 0; i < n; i++)
+            time_step_kernel(&rk, l->step_factors + i, l->fluxes + 5 * (size_t)i, l->old_variables + 5 * (size_t)i,
+                             l->variables + 5 * (size_t)i);
+    }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int i = 0; i < n; i++)
+        residual_kernel(l->old_variables + 5 * (size_t)i, l->variables + 5 * (size_t)i, l->residuals + 5 * (size_t)i);
+}
+
+int ref_mg_run_omp(void* plan, orc_level* L, int nlev, int cycles, const orc_consts* c, double* rms_hist,
+                   int threads)
+{
+    const RefOmpPlan& P = *static_cast<const RefOmpPlan*>(plan);
+    load_consts(c);
+    int level = 0, mg_dir = MG_UP, i = 0;
+    while (i < cycles) {
+        orc_level* l = &L[level];
+        const RefOmpLevel& q = P.L[level];
+        const int n = l->n_nodes;
+        double min_dt;
+        omp_smooth_level(q, l, threads, &min_dt);
+        if (min_dt < 0.0f) return 1;
+        omp_stages(q, l, min_dt, threads);
+        if (level == 0) {
+            double rms = 0.0;
+            int bad = 0;
+#pragma omp parallel num_threads(threads)
+            {
+                double rt = 0.0;
+                int bt = 0;
+#pragma omp for schedule(static)
+                for (int j = 0; j < n; j++) {
+                    calc_rms_kernel(l->residuals + 5 * (size_t)j, &rt);
+                    count_bad_vals(l->variables + 5 * (size_t)j, &bt);
+                }
+#pragma omp critical
+                {
+                    rms += rt;
+                    bad += bt;
+                }
+            }
+            if (rms_hist) rms_hist[i] = sqrt(rms / double(n));
+            if (bad > 0) return 2;
+        }
+        if (nlev <= 1) {
+            i++;
+        } else if (mg_dir == MG_UP) {
+            level++;
+            orc_level* f = &L[level - 1];
+            orc_level* cl = &L[level];
+            const RefOmpLevel& fq = P.L[level - 1];
+            const int ncn = cl->n_nodes;
+            // up_pre + up + up_post (mg.h:29-64) per coarse node over its children in fine order
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int cc = 0; cc < ncn; cc++) {
+                const size_t cv = (size_t)cc;
+                if (fq.fptr[cv + 1] == fq.fptr[cv]) continue;
+                up_pre_kernel(cl->variables + 5 * cv, cl->up_scratch + cv);
+                for (int j = fq.fptr[cv]; j < fq.fptr[cv + 1]; j++)
+                    up_kernel(f->variables + 5 * (size_t)fq.fine[j], cl->variables + 5 * cv, cl->up_scratch + cv);
+            }
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int cc = 0; cc < ncn; cc++) up_post_kernel(cl->variables + 5 * (size_t)cc, cl->up_scratch + cc);
+            if (level == nlev - 1) mg_dir = MG_DOWN;
+        } else {
+            level--;
+            orc_level* f = &L[level];
+            orc_level* cl = &L[level + 1];
+            const int nf = f->n_nodes;
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int j = 0; j < nf; j++) {
+                const size_t fi = (size_t)j, cc = (size_t)f->node_to_mg_node[j];
+                down_kernel(f->variables + 5 * fi, f->residuals + 5 * fi, f->coords + 3 * fi, cl->residuals + 5 * cc,
+                            cl->coords + 3 * cc);
+            }
+            if (level == 0) { mg_dir = MG_UP; i++; }
+        }
+    }
     return 0;
 }
 

@@ -1,0 +1,367 @@
+"""GPU parity at the full size of each benchmark workload.
+
+The other GPU suites check every loop and every driver form on meshes of up
+to 12K nodes; this one runs the five workloads the benchmark quotes, at their
+own sizes, against the CPU oracle where it finishes in seconds and through
+size-independent properties where it does not:
+
+  box     40^3 jittered box, 4 levels (64K/8K/1K/125 nodes), M0.5 a3deg,
+          10 cycles: every variant and driver form against the oracle.
+  knn300k 300K-node / 930K-edge kNN mesh, 4 levels (300K/165K/111K/81K),
+          M0.5 a3deg: 10 and 50 cycles against the oracle.
+  knn8m   8M-node / 24.8M-edge kNN mesh, 4 levels, M0.7 a2deg, 50 cycles:
+          the FP64 part variants and mixed precision against the exact FP64
+          GPU solve (which the two workloads above pin to the oracle).
+  batch64 64 meshes, N log-uniform in [20K, 500K], 4 levels, Mach U(0.3,
+          0.8), angle U(-2, 6) deg, 30 cycles, batched: every mesh against
+          its own FP64 solve alone, the two smallest also against the oracle.
+  served  the first six requests of the served stream (N lognormal in
+          [10K, 2M], FP64 / mixed 50/50, 1e-6 RMS drop or 200 cycles): each
+          result in the mix bitwise equal to the request served alone, the
+          smallest also against the oracle.
+
+Bars (relative, per variable, max over nodes / max |oracle|): FP64 variants
+1e-12 at every level; mixed precision (FP32 flux arithmetic, FP64
+accumulation) 1e-6.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import mgcfd
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STATE_TOL = 1e-12
+MIXED_STATE_TOL = 1e-6
+# The L0 RMS residual is |var - old| summed, and var - old is ~1e-6 of var
+# per cycle here (damped weights), so a state difference of ~1e-16 relative
+# shows up ~1e6 times larger in it: FP64 RMS histories are held to 1e-8,
+# mixed precision to 1e-3, relative per cycle.
+RMS_TOL = 1e-8
+MIXED_RMS_TOL = 1e-3
+
+
+def rel_per_var(g, o):
+    return np.abs(np.asarray(g) - o).max(axis=0) / np.maximum(np.abs(o).max(axis=0), 1e-300)
+
+
+def rel(g, o):
+    return float(rel_per_var(g, o).max())
+
+
+def tol_of(variant):
+    return MIXED_STATE_TOL if variant.startswith("mixed") else STATE_TOL
+
+
+@pytest.fixture
+def rt():
+    r = mgcfd.Runtime()
+    r.set_flux_variant(mgcfd.FLUX_VARIANTS["gather_exact"])
+    yield r
+    r.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle("orc")
+
+
+# ------------------------------------------------------------ 40^3 box --
+
+@pytest.fixture(scope="module")
+def box40_ref(box40, orc):
+    return orc.solve(box40, 10, 0.5, 3.0)
+
+
+BOX_CASES = [("gather_exact", "loops"), ("gather_exact", "device"), ("gather_exact", "graph"),
+             ("gather_fast", "graph"), ("part_coloured", "device"), ("part_coloured", "graph"),
+             ("part_atomic", "device"), ("part_atomic", "graph"), ("mixed_coloured", "graph"),
+             ("mixed_atomic", "graph")]
+
+
+@pytest.mark.parametrize("variant,mode", BOX_CASES)
+def test_box40_solve_vs_oracle(rt, box40, box40_ref, variant, mode):
+    s = mgcfd.Solver(rt, box40, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.setup()
+    if mode == "loops":
+        st, rms, _ = s.run_loops(10)
+    else:
+        st, rms = s.run_device(10, use_graph=(mode == "graph"))
+    assert st == 0 and box40_ref["status"] == 0
+    errs = [rel(s.state(l), box40_ref["levels"][l]["variables"]) for l in range(len(box40))]
+    print(f"box40 {variant}/{mode}: per-level state error {errs}")
+    assert max(errs) <= tol_of(variant)
+    rms_tol = RMS_TOL if tol_of(variant) == STATE_TOL else MIXED_RMS_TOL
+    assert np.allclose(rms, box40_ref["rms"], rtol=rms_tol, atol=0)
+
+
+# ------------------------------------------------- 300K-node kNN mesh --
+
+@pytest.fixture(scope="module")
+def knn300k():
+    return mgcfd.mesh_knn(300_000, 930_000, 4, seed=2)
+
+
+@pytest.fixture(scope="module")
+def knn300k_ref(knn300k, orc):
+    t = time.time()
+    r = orc.solve(knn300k, 50, 0.5, 3.0, checkpoints=[10])
+    print(f"oracle: 50 cycles of the 300K-node mesh in {time.time() - t:.1f} s")
+    assert r["status"] == 0
+    return r
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("variant", ["gather_exact", "part_coloured", "part_atomic", "mixed_atomic"])
+def test_knn300k_solve_vs_oracle_10_and_50_cycles(rt, knn300k, knn300k_ref, variant):
+    s = mgcfd.Solver(rt, knn300k, 0.5, 3.0)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    s.setup()
+    st, rms10 = s.run_device(10, use_graph=True)
+    assert st == 0
+    errs10 = [rel(s.state(l), knn300k_ref["snapshots"][10][l]) for l in range(4)]
+    st, rms40 = s.run_device(40, use_graph=True)      # continues from cycle 10
+    assert st == 0
+    errs50 = [rel(s.state(l), knn300k_ref["levels"][l]["variables"]) for l in range(4)]
+    print(f"knn300k {variant}: per-level state error after 10 cycles {errs10}, after 50 {errs50}")
+    print(f"knn300k {variant}: per-variable L0 error after 50 cycles "
+          f"{rel_per_var(s.state(0), knn300k_ref['levels'][0]['variables'])}")
+    assert max(errs10) <= tol_of(variant) and max(errs50) <= tol_of(variant)
+    rms = np.concatenate([rms10, rms40])
+    rms_err = np.abs(rms - knn300k_ref["rms"]) / np.abs(knn300k_ref["rms"])
+    print(f"knn300k {variant}: largest relative RMS-history difference {rms_err.max():.3e}")
+    assert rms_err.max() <= (RMS_TOL if tol_of(variant) == STATE_TOL else MIXED_RMS_TOL)
+
+
+@pytest.mark.timeout(900)
+def test_knn300k_loop_by_loop_vs_oracle(rt, knn300k, knn300k_ref):
+    """The op_par_loop path (host-driven, one loop at a time) over 10 cycles."""
+    s = mgcfd.Solver(rt, knn300k, 0.5, 3.0)
+    s.setup()
+    st, rms, _ = s.run_loops(10)
+    assert st == 0
+    errs = [rel(s.state(l), knn300k_ref["snapshots"][10][l]) for l in range(4)]
+    print(f"knn300k loops: per-level state error after 10 cycles {errs}")
+    assert max(errs) <= STATE_TOL
+    assert np.allclose(rms, knn300k_ref["rms"][:10], rtol=RMS_TOL, atol=0)
+
+
+# --------------------------------------------------- 8M-node kNN mesh --
+
+@pytest.fixture(scope="module")
+def knn8m():
+    t = time.time()
+    m = mgcfd.mesh_knn(8_000_000, 24_800_000, 4, seed=8)
+    print(f"8M-node mesh: {[len(l['coords']) for l in m]} nodes per level, {time.time() - t:.1f} s")
+    return m
+
+
+# References computed on the GPU with the runtime of the first test that
+# needs them (one library instance at a time), kept for the module.
+_cache = {}
+
+
+def knn8m_exact(rt, mesh):
+    if "knn8m" not in _cache:
+        b = mgcfd.Batch(rt, [mesh], [0.7], [2.0], "gather_exact")
+        st, rms = b.run(50)
+        assert (st == 0).all()
+        _cache["knn8m"] = ([b.fetch(0, l) for l in range(4)], rms[0])
+        b.close()
+    return _cache["knn8m"]
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("variant", ["part_atomic", "part_coloured", "mixed_atomic", "mixed_coloured"])
+def test_knn8m_variants_vs_exact_fp64(rt, knn8m, variant):
+    """At full size the oracle would take hours, so the exact FP64 GPU solve
+    (bitwise the oracle's flux arithmetic, pinned at the two sizes above) is
+    the reference: FP64 part variants within 1e-12, mixed within 1e-6."""
+    ref, ref_rms = knn8m_exact(rt, knn8m)
+    b = mgcfd.Batch(rt, [knn8m], [0.7], [2.0], variant)
+    st, rms = b.run(50)
+    assert (st == 0).all()
+    errs = []
+    for l in range(4):
+        e = rel_per_var(b.fetch(0, l), ref[l])
+        errs.append(float(e.max()))
+        if l == 0:
+            print(f"knn8m {variant}: per-variable L0 error vs exact FP64 {e}")
+    b.close()
+    print(f"knn8m {variant}: per-level state error vs exact FP64 {errs}")
+    assert max(errs) <= tol_of(variant)
+    rms_err = float((np.abs(rms[0] - ref_rms) / np.abs(ref_rms)).max())
+    print(f"knn8m {variant}: largest relative RMS-history difference {rms_err:.3e}")
+    assert rms_err <= (RMS_TOL if tol_of(variant) == STATE_TOL else MIXED_RMS_TOL)
+
+
+# -------------------------------------------------------- 64-mesh batch --
+
+@pytest.fixture(scope="module")
+def batch64():
+    rng = np.random.default_rng(40)
+    n = 64
+    sizes = np.exp(rng.uniform(np.log(20_000), np.log(500_000), n)).astype(int)
+    mach = rng.uniform(0.3, 0.8, n)
+    aoa = rng.uniform(-2.0, 6.0, n)
+    meshes = [mgcfd.mesh_knn(int(k), int(round(3.1 * k)), 4, seed=1000 + i) for i, k in enumerate(sizes)]
+    return meshes, mach, aoa
+
+
+def batch64_alone(rt, batch64):
+    """Each mesh's own FP64 solve (exact variant, a batch of one)."""
+    if "batch64" not in _cache:
+        meshes, mach, aoa = batch64
+        out = []
+        for i, m in enumerate(meshes):
+            b = mgcfd.Batch(rt, [m], [mach[i]], [aoa[i]], "gather_exact")
+            st, _ = b.run(30)
+            assert (st == 0).all()
+            out.append([b.fetch(0, l) for l in range(4)])
+            b.close()
+        _cache["batch64"] = out
+    return _cache["batch64"]
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("variant", ["mixed_coloured", "part_coloured"])
+def test_batch64_each_mesh_vs_its_own_solve(rt, batch64, variant):
+    meshes, mach, aoa = batch64
+    alone = batch64_alone(rt, batch64)
+    b = mgcfd.Batch(rt, meshes, mach, aoa, variant)
+    part = b.autotune(3)
+    st, rms = b.run(30)
+    assert (st == 0).all()
+    worst = 0.0
+    for i in range(len(meshes)):
+        for l in range(4):
+            worst = max(worst, rel(b.fetch(i, l), alone[i][l]))
+    b.close()
+    print(f"batch64 {variant} (part size {part}): worst per-level state error over 64 meshes {worst:.3e}")
+    assert worst <= tol_of(variant)
+
+
+@pytest.mark.timeout(900)
+def test_batch64_smallest_meshes_vs_oracle(rt, orc, batch64):
+    meshes, mach, aoa = batch64
+    alone = batch64_alone(rt, batch64)
+    order = np.argsort([len(m[0]["coords"]) for m in meshes])[:2]
+    b = mgcfd.Batch(rt, meshes, mach, aoa, "mixed_coloured")
+    b.run(30)
+    for i in order:
+        ref = orc.solve(meshes[i], 30, mach[i], aoa[i])
+        assert ref["status"] == 0
+        e_alone = max(rel(alone[i][l], ref["levels"][l]["variables"]) for l in range(4))
+        e_mixed = max(rel(b.fetch(i, l), ref["levels"][l]["variables"]) for l in range(4))
+        print(f"batch64 mesh {i} ({len(meshes[i][0]['coords'])} nodes): FP64 alone {e_alone:.3e}, "
+              f"mixed in the batch {e_mixed:.3e} vs the oracle")
+        assert e_alone <= STATE_TOL and e_mixed <= MIXED_STATE_TOL
+    b.close()
+
+
+# ------------------------------------------------------- served stream --
+
+@pytest.fixture(scope="module")
+def served_requests():
+    rng = np.random.default_rng(50)
+    n_req = 32
+    sizes = np.clip(np.exp(rng.normal(np.log(100_000), 1.0, n_req)), 10_000, 2_000_000).astype(int)
+    mach = rng.uniform(0.3, 0.8, n_req)
+    aoa = rng.uniform(-2.0, 6.0, n_req)
+    variant = np.where(rng.uniform(size=n_req) < 0.5, "part_coloured", "mixed_coloured")
+    k = 6
+    meshes = [mgcfd.mesh_knn(int(n), int(round(3.1 * n)), 4, seed=2000 + i) for i, n in enumerate(sizes[:k])]
+    return [(meshes[i], float(mach[i]), float(aoa[i]), str(variant[i])) for i in range(k)]
+
+
+def drain(srv, ids, timeout=600.0):
+    t0 = time.time()
+    out = {}
+    while len(out) < len(ids):
+        srv.progress()
+        for i in ids:
+            if i not in out:
+                r = srv.result(i)
+                if r is not None:
+                    out[i] = r
+        assert time.time() - t0 < timeout, "served requests did not finish"
+        time.sleep(0.001)
+    return out
+
+
+@pytest.mark.timeout(900)
+def test_served_stream_mix_equals_alone(rt, orc, served_requests):
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=50)
+    ids = [srv.submit(m, mach, aoa, var, max_cycles=200, rms_drop=1e-6) for m, mach, aoa, var in served_requests]
+    mix = drain(srv, ids)
+    spans = [srv.timing(i) for i in ids]
+    srv.close()
+    print("served: device intervals (ms)", [(round(a, 1), round(b, 1)) for a, b in spans])
+    for k, (m, mach, aoa, var) in enumerate(served_requests):
+        r = mix[ids[k]]
+        assert r["status"] in (0, 3)
+        alone = mgcfd.Server(rt, budget_bytes=200e9, max_active=1, chunk=200)
+        j = alone.submit(m, mach, aoa, var, max_cycles=200, rms_drop=1e-6)
+        a = drain(alone, [j])[j]
+        alone.close()
+        print(f"request {k}: {len(m[0]['coords'])} nodes, {var}, {r['cycles']} cycles, status {r['status']}, "
+              f"latency {r['latency']:.3f} s")
+        assert a["cycles"] == r["cycles"] and a["status"] == r["status"]
+        assert np.array_equal(a["variables"].view(np.uint64), r["variables"].view(np.uint64)), k
+    # the smallest request against the oracle for the cycles it ran
+    k = int(np.argmin([len(q[0][0]["coords"]) for q in served_requests]))
+    m, mach, aoa, var = served_requests[k]
+    r = mix[ids[k]]
+    ref = orc.solve(m, r["cycles"], mach, aoa)
+    err = rel(r["variables"], ref["levels"][0]["variables"])
+    print(f"request {k} ({var}) vs the oracle after {r['cycles']} cycles: {err:.3e}")
+    assert err <= tol_of(var)
+
+
+# ------------------------------------------ spread of the atomic variants --
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("variant", ["part_atomic", "mixed_atomic", "atomic"])
+def test_knn300k_atomic_spread(rt, orc, knn300k, knn300k_ref, variant):
+    """The atomic variants add each node's increments in whatever order the
+    hardware completes them.  Their run-to-run spread is measured beside
+    their difference from the oracle: 8 flux passes over level 0 of the
+    300K-node mesh from one perturbed state, and two 10-cycle solves."""
+    from conftest import perturbed_state
+    s = mgcfd.Solver(rt, knn300k, 0.5, 3.0)
+    s.setup()
+    c = orc.freestream(0.5, 3.0)
+    n = len(knn300k[0]["coords"])
+    var = perturbed_state(n, c.ff_variable[:], seed=7)
+    w = s.state(0, "edge_weights")           # the setup weights, bitwise the oracle's
+    o = np.zeros((n, 5))
+    orc.compute_flux_edge(np.ascontiguousarray(knn300k[0]["edge_to_node"], np.int32), var, w, o, c)
+    rt.upload(s.lv[0].variables, var)
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    passes = []
+    for _ in range(8):
+        rt.upload(s.lv[0].fluxes, np.zeros((n, 5)))
+        s.flux_edge(0)
+        passes.append(s.state(0, "fluxes"))
+    flux_spread = max(rel(p, passes[0]) for p in passes[1:])
+    flux_vs_oracle = max(rel(p, o) for p in passes)
+    n_differ = sum(not np.array_equal(p.view(np.uint64), passes[0].view(np.uint64)) for p in passes[1:])
+    print(f"{variant} flux pass: run-to-run spread {flux_spread:.3e} ({n_differ} of 7 passes differ bitwise "
+          f"from the first), vs oracle {flux_vs_oracle:.3e}")
+    assert flux_spread <= (1e-4 if variant.startswith("mixed") else 1e-12)
+    assert flux_vs_oracle <= (1e-4 if variant.startswith("mixed") else 1e-12)
+    states = []
+    for _ in range(2):
+        t = mgcfd.Solver(rt, knn300k, 0.5, 3.0)
+        t.setup()
+        st, _ = t.run_device(10, use_graph=True)
+        assert st == 0
+        states.append([t.state(l) for l in range(4)])
+    solve_spread = max(rel(states[1][l], states[0][l]) for l in range(4))
+    solve_vs_oracle = max(rel(states[k][l], knn300k_ref["snapshots"][10][l]) for k in range(2) for l in range(4))
+    print(f"{variant} 10-cycle solve: run-to-run spread {solve_spread:.3e}, vs oracle {solve_vs_oracle:.3e}")
+    assert solve_spread <= tol_of(variant) and solve_vs_oracle <= tol_of(variant)

@@ -81,11 +81,46 @@ def test_served_results_match_oracle_and_alone(rt, meshes):
 
 
 def test_memory_budget_serialises(rt, meshes):
-    srv = mgcfd.Server(rt, budget_bytes=1.0, max_active=4, chunk=2)   # room for one request at a time
+    est = [mgcfd.Server.estimate_bytes(meshes[i]) for i in range(3)]
+    # room for the largest request, never for two at once
+    budget = max(est) + 0.5 * min(est)
+    srv = mgcfd.Server(rt, budget_bytes=budget, max_active=4, chunk=2)
     ids = [srv.submit(meshes[i], 0.5, 3.0, "gather_exact", max_cycles=3) for i in range(3)]
     res = drain(srv, ids, max_active=1)
     srv.close()
     assert all(res[i]["status"] == 0 for i in ids)
+
+
+def test_request_over_budget_refused(rt, meshes):
+    """A request whose estimate exceeds the whole budget is refused at submit
+    (it could never be admitted), and the server keeps serving the others."""
+    est = mgcfd.Server.estimate_bytes(meshes[0])
+    srv = mgcfd.Server(rt, budget_bytes=0.5 * est, max_active=2, chunk=2)
+    with pytest.raises(mgcfd.MgcfdError, match="more than the whole budget"):
+        srv.submit(meshes[0], 0.5, 3.0, "gather_exact", max_cycles=3)
+    small = mgcfd.mesh_knn(300, 930, 3, seed=15)
+    assert mgcfd.Server.estimate_bytes(small) <= 0.5 * est
+    j = srv.submit(small, 0.5, 3.0, "gather_exact", max_cycles=3)
+    r = drain(srv, [j])[j]
+    srv.close()
+    assert r["status"] == 0 and r["cycles"] == 3
+
+
+def test_admitted_requests_overlap_on_device(rt, meshes):
+    """Requests admitted together run concurrently: their device intervals
+    (HIP events on each request's stream) intersect."""
+    big = [mgcfd.mesh_knn(20000, 62000, 3, seed=16 + k) for k in range(3)]
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=4, chunk=400)
+    # all three are queued before the first progress call admits them
+    ids = [srv.submit(m, 0.5, 3.0, "part_coloured", max_cycles=400) for m in big]
+    res = drain(srv, ids, max_active=4)
+    spans = [srv.timing(i) for i in ids]
+    srv.close()
+    print("device intervals (ms):", spans)
+    assert all(res[i]["status"] == 0 for i in ids)
+    assert all(0 <= a < b for a, b in spans)
+    # the later requests started on the device before the first one ended
+    assert spans[1][0] < spans[0][1] and spans[2][0] < spans[0][1]
 
 
 def test_convergence_stop_is_exact(rt, meshes):
