@@ -204,28 +204,23 @@ def cpu_solve(mesh, host, cycles=10):
     """The whole 4-level solve of the same mesh on the host with the
     reference's kernels (oracle/_ref): 'cycles' cycles with every loop
     spread over the baseline's threads as OP2's OpenMP backend runs them (on
-    the mesh renumbered as op_renumber would), and one cycle in OP2
-    sequential order on one thread (a 1-cycle solve minus a 0-cycle one)."""
+    the mesh renumbered as op_renumber would), and two cycles of the same on
+    one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     if not Oracle.available("ref"):
         return None
     ref = Oracle("ref")
     omp = ref.solve_omp(mesh, cycles, MACH, AOA, threads=host["threads"])
-    t = time.perf_counter()
-    ref.solve(mesh, 0, MACH, AOA)
-    t0 = time.perf_counter() - t
-    t = time.perf_counter()
-    one = ref.solve(mesh, 1, MACH, AOA)
-    t1 = time.perf_counter() - t
-    # the threaded run against the sequential one's first cycle, as a sanity check of the threading
+    one = ref.solve_omp(mesh, 2, MACH, AOA, threads=1)
     return dict(ms_per_cycle=round(omp["seconds"] / cycles * 1e3, 2), cycles=cycles, cores=host["threads"],
-                status=omp["status"], single_thread_ms_per_cycle=round((t1 - t0) * 1e3, 1), kind="reference",
+                status=omp["status"], single_thread_ms_per_cycle=round(one["seconds"] / 2 * 1e3, 1),
+                kind="reference",
                 sample=f"reference kernels (oracle/_ref): {cycles} cycles of the {len(mesh[0]['coords'])}-node "
-                       f"4-level solve on {host['threads']} threads (OpenMP, edge blocks coloured as OP2's OpenMP "
-                       f"backend colours them, mesh renumbered as op_renumber would; {omp['seconds']:.2f} s) and "
-                       f"one cycle in OP2 sequential order on one thread ({t1 - t0:.2f} s); {host['model']}",
-                rms_rel_diff_vs_sequential=float(f"{abs(omp['rms'][0] / one['rms'][0] - 1):.3e}"))
+                       f"4-level solve on {host['threads']} threads ({omp['seconds']:.2f} s) and 2 cycles on one "
+                       f"thread ({one['seconds']:.2f} s), loops spread as OP2's OpenMP backend spreads them (edge "
+                       f"blocks coloured) over the mesh renumbered as op_renumber would; {host['model']}",
+                rms_rel_diff_one_vs_many_threads=float(f"{abs(omp['rms'][0] / one['rms'][0] - 1):.3e}"))
 
 
 def bench_large(rt, cycles=50, n=8_000_000, mach=0.7, aoa=2.0):
