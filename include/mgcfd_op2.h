@@ -262,6 +262,12 @@ int mgcfd_plan_stats(op_map map, int* stats);
  * [5] dynamic LDS bytes per workgroup of the FP64 kernel. */
 int mgcfd_part_plan_stats(op_map map, int* stats);
 
+/* Part plan configuration of an edge->node map (variants PART_* / MIXED_*):
+ * part_nodes in [16, 256] owned nodes per part; part_threads 0 (default),
+ * 256 or 512 for the kernel with the fused Runge-Kutta stage; record_bytes
+ * 0 (by the plan's rule), 40 or 48 for the FP64 edge-flux kernel's LDS node
+ * records.  Re-plans the map. */
+int mgcfd_map_part_config(op_map map, int part_nodes, int part_threads, int record_bytes);
 /* Last error message of this thread ("" if none). */
 const char* mgcfd_last_error(void);
 
@@ -329,10 +335,19 @@ typedef struct mgcfd_batch_s* mgcfd_batch;
  * (euler3d.cpp:412-441) for every mesh. */
 int mgcfd_batch_create(int n, const mgcfd_mesh* meshes, const double* mach, const double* aoa_deg,
                        int flux_variant, mgcfd_batch* out);
-/* Autotuner: times the level-0 flux of the batch for every candidate part
- * size (nodes per workgroup of the part variants) and keeps the fastest for
- * all levels; *part_nodes receives it. */
+/* Autotuner of the part variants' kernels (the edge flux with its fused
+ * Runge-Kutta stage, which every cycle runs): per level, the nodes per part
+ * (256, 192 or 128) and threads per part (256 or 512) that give the fastest
+ * captured multigrid cycle of the whole batch ('iters' timed cycles per
+ * candidate, levels tuned in turn).  The batch's state is left as it was.
+ * part_nodes[levels] / part_threads[levels] (nullable) receive the choice;
+ * variants without a part plan have nothing to tune (0s).
+ * mgcfd_batch_autotune is the same tuning, reporting level 0's part size. */
 int mgcfd_batch_autotune(mgcfd_batch b, int iters, int* part_nodes);
+int mgcfd_batch_autotune_levels(mgcfd_batch b, int iters, int* part_nodes, int* part_threads);
+/* Sets one level's part configuration directly (a tuning found earlier):
+ * part_nodes in [16, 256], part_threads 256 or 512.  Re-plans the level. */
+int mgcfd_batch_set_part_config(mgcfd_batch b, int level, int part_nodes, int part_threads);
 /* Runs 'cycles' multigrid cycles from the setup state; rms receives
  * [n][cycles] L0 RMS residuals and status [n] (0 ok, 1 negative min dt,
  * 2 NaN/Inf; either may be NULL). */

@@ -947,6 +947,12 @@ __global__ void k_fill(size_t n, double* __restrict__ x, double val)
     if (i < n) x[i] = val;
 }
 
+__global__ void k_fill_u64(size_t n, unsigned long long* __restrict__ x, unsigned long long val)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = val;
+}
+
 __global__ void k_fill_i(size_t n, int* __restrict__ x, int val)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1034,41 +1040,41 @@ __device__ __forceinline__ int block_sum_i(int x)
 
 // Publication of a level's minimum dt by the kernels that compute dt
 // (get_min_dt_kernel, time_stepping_kernels.h:34-41, fused into them): every
-// workgroup writes its minimum to part[blockIdx.x] and counts itself in its
-// mesh's arrival counter; the mesh's last workgroup to arrive reduces the
-// mesh's minima, writes the result to out[seg] and resets the counter, so
-// the counters are zero again for the next launch (also in a replayed
-// graph).  min is exact: the order of arrival does not change the result.
+// workgroup folds its minimum into one of kMinSlots slots of its mesh with a
+// 64-bit atomic min on an order-preserving integer image of the double, so
+// the first Runge-Kutta stage reduces kMinSlots values instead of one per
+// workgroup of the level.  The atomics need no fence (the stage is the next
+// kernel on the stream), and spreading the workgroups over slots keeps them
+// off one address.  The second stage resets the slots to the empty value
+// for the next smoothing step.  min is exact: the order of the atomics does
+// not change the result.
+constexpr int kMinSlots = 64;
+constexpr unsigned long long kMinEmpty = 0xFFEFFFFFFFFFFFFFull;    // min_key(DBL_MAX)
+
+// Order-preserving image of a double (not NaN): unsigned order of the images
+// is the order of the values; -0.0 and +0.0 share the image of +0.0.
+__device__ __forceinline__ unsigned long long min_key(double d)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(d == 0.0 ? 0.0 : d);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double min_unkey(unsigned long long k)
+{
+    return __longlong_as_double((long long)((k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
+}
+
 struct MinPub {
-    const int* sblk;    // first block of each segment (null: one segment of gridDim.x blocks)
-    unsigned* cnt;      // arrival counter per segment, zero between launches
-    double* out;        // minimum per segment
+    const int* sblk;            // first block of each segment (null: one segment of gridDim.x blocks)
+    unsigned long long* slots;  // kMinSlots per segment
 };
 
 // m: the workgroup's minimum (valid in thread 0).
-template <int NT = kBlock>
-__device__ __forceinline__ void min_publish(const MinPub& p, double* __restrict__ part, int seg, double m)
+__device__ __forceinline__ void min_publish(const MinPub& p, int seg, double m)
 {
-    __shared__ int last;
-    const int lo = p.sblk ? p.sblk[seg] : 0, hi = p.sblk ? p.sblk[seg + 1] : (int)gridDim.x;
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = m;
-        __threadfence();
-        last = atomicAdd(p.cnt + seg, 1u) == (unsigned)(hi - lo - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double x = DBL_MAX;
-    for (int j = lo + (int)threadIdx.x; j < hi; j += NT) {
-        const double y = __longlong_as_double((long long)__hip_atomic_load(
-            reinterpret_cast<unsigned long long*>(part + j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        x = (y < x) ? y : x;
-    }
-    x = block_min<NT>(x);
-    if (threadIdx.x == 0) {
-        p.out[seg] = x;
-        __hip_atomic_store(p.cnt + seg, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0 && p.slots) {
+        const int lo = p.sblk ? p.sblk[seg] : 0;
+        atomicMin(p.slots + (size_t)seg * kMinSlots + ((int)blockIdx.x - lo) % kMinSlots, min_key(m));
     }
 }
 
@@ -1116,21 +1122,19 @@ struct DtArgs {
     const double* vol;
     double* dt;
     double* old;
-    double* part;
     MinPub pub;
 };
 
 __global__ __launch_bounds__(kBlock) void k_calc_dt(
     int n, const double* __restrict__ var, const double* __restrict__ vol,
-    double* __restrict__ dt, double* __restrict__ old, double* __restrict__ part,
-    const int* __restrict__ status, MinPub pub)
+    double* __restrict__ dt, double* __restrict__ old, const int* __restrict__ status, MinPub pub)
 {
 #pragma clang fp contract(off)
     if (halted(status)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = DBL_MAX;
     if (i < n) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
-    if (part) min_publish(pub, part, 0, block_min(d));
+    if (pub.slots) min_publish(pub, 0, block_min(d));
 }
 
 // get_min_dt_kernel as a standalone loop: per-workgroup min of dt.
@@ -1205,7 +1209,8 @@ struct StageArgs {
     int n, rk;
     const int4* blk;           // batched solves: {segment, first, end} per workgroup; null: plain blocks over n
     const int* sblk;           // batched solves: first workgroup of each segment
-    const double* mdt;         // first stage: minimum dt published by the dt kernel: [0], or per segment
+    unsigned long long* mslots;   // minimum-dt slots (kMinSlots per segment): read by the first stage,
+    int nmslots;                  // reset by the second
     int* mstatus;              // abort code 1 on a negative minimum: one status, or per segment
     int* zero;                 // first stage: a counter the last stage accumulates into, cleared
     const double* vol;
@@ -1224,18 +1229,31 @@ struct StageArgs {
     const int* status;
 };
 
-// First stage: the minimum dt of the workgroup's mesh, as the kernel that
-// computed dt published it (min_publish); the mesh's first workgroup raises
-// abort code 1 on a negative value (euler3d.cpp:476-484).
+// First stage: the minimum dt of the workgroup's mesh, reduced from the
+// mesh's slots that the kernel computing dt filled (min_publish); the mesh's
+// first workgroup raises abort code 1 on a negative value
+// (euler3d.cpp:476-484).  Second stage: the slots are reset for the next
+// smoothing step.
 template <int NT>
 __device__ __forceinline__ double stage_min_dt(const StageArgs& a, int seg, bool first)
 {
-    const double mdt = a.mdt[seg];
+    static_assert(NT >= kMinSlots, "one slot per thread");
+    const unsigned long long k =
+        threadIdx.x < kMinSlots ? a.mslots[(size_t)seg * kMinSlots + threadIdx.x] : kMinEmpty;
+    const double m = block_min_all<NT>(min_unkey(k));
+    const double mdt = (m < DBL_MAX) ? m : DBL_MAX;
     if (first && threadIdx.x == 0) {
         if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
         if (a.zero) *a.zero = 0;
     }
     return mdt;
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_reset_min(const StageArgs& a)
+{
+    if (a.rk == 1 && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < a.nmslots; i += NT) a.mslots[i] = kMinEmpty;
 }
 
 // One node of a Runge-Kutta stage, in the reference's order: f holds node
@@ -1338,6 +1356,7 @@ __global__ void k_pack_bnd(int nq, const int* __restrict__ bent, const double* _
 __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 {
     if (halted(a.status)) return;
+    stage_reset_min<kBlock>(a);
     int i, lim, seg = 0;
     bool first;                // first workgroup of its mesh
     if (a.blk) {
@@ -1385,6 +1404,7 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
 {
     extern __shared__ double sm[];
     if (halted(a.status)) return;
+    stage_reset_min<T>(a);
     const int p = xcd_block();
     const PartInfo P = parts[p];
     const int seg = pseg ? pseg[p] : 0;
@@ -1670,7 +1690,7 @@ __global__ __launch_bounds__(kBlock) void k_up_gather_dt(
         }
         m = min_candidate(calc_dt_state(c, acc, d.vol, d.dt, d.old));
     }
-    min_publish(d.pub, d.part, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
+    min_publish(d.pub, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
 }
 
 __global__ void k_down(int nf, const int* __restrict__ f2c, double* __restrict__ vf,
@@ -1812,7 +1832,7 @@ __global__ __launch_bounds__(kBlock) void k_down_dt(
         store5(vf, i, u);
         m = min_candidate(calc_dt_state(i, u, d.vol, d.dt, d.old));
     }
-    min_publish(d.pub, d.part, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
+    min_publish(d.pub, d.blk ? d.blk[blockIdx.x].x : 0, block_min(m));
 }
 
 // ------------------------------------------------------------------------
@@ -1872,15 +1892,15 @@ __global__ void k_cell_volumes_node(int n_targets, const int* __restrict__ targe
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_calc_dt_seg(const int4* __restrict__ blk, const double* __restrict__ var,
                                                         const double* __restrict__ vol, double* __restrict__ dt,
-                                                        double* __restrict__ old, double* __restrict__ part,
-                                                        const int* __restrict__ halt, MinPub pub)
+                                                        double* __restrict__ old, const int* __restrict__ halt,
+                                                        MinPub pub)
 {
     if (halted(halt)) return;
     const int4 B = blk[blockIdx.x];
     const int i = B.y + threadIdx.x;
     double d = DBL_MAX;
     if (i < B.z) d = min_candidate(calc_dt_node(i, var, vol, dt, old));
-    min_publish(pub, part, B.x, block_min(d));
+    min_publish(pub, B.x, block_min(d));
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,

@@ -117,6 +117,8 @@ struct InvPlan {
     unsigned pt_w32ver = 0;
     size_t pt_lds32 = 0;
     int pt_psize = kPart;            // max nodes per part (autotuned)
+    int pt_threads = 0;              // threads per part of the kernel with the fused stage (0: default; autotuned)
+    int pt_rs_set = 0;               // FP64 record values of the edge-flux kernel: 0 by the rule below, or 5 / 6
     bool bk_built = false, pt_built = false;
     std::vector<int> h_ptr;          // host CSR kept for re-planning (dim-2 dense maps)
     std::vector<int2> h_inc2;
@@ -900,6 +902,7 @@ static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, con
     // shortens; else 48 bytes (the speed formed once per staged node).
     const bool lds_bound = (size_t)rec_cap * kPRec * sizeof(double) + (size_t)kPart * 5 * sizeof(double) > 40960;
     P.pt_rs = (lds_bound || (long long)parts.size() < 4LL * R.n_cu) ? 5 : kPRec;
+    if (P.pt_rs_set) P.pt_rs = P.pt_rs_set;
     P.pt_lds = sizeof(double) * ((size_t)rec_cap * P.pt_rs + (size_t)kPart * 5);
     P.pt_lds32 = sizeof(double) * (((size_t)rec_cap * PartRec<float>::kStride * sizeof(float) + 7) / 8 +
                                    (size_t)kPart * 5);   // FP32 records, FP64 accumulators
@@ -1277,16 +1280,18 @@ static int launch_part_n(bool coloured, const InvPlan& P, const Real* w, size_t 
 // workgroup's edge phase; the kernel with the fused stage needs ~96 VGPRs
 // (its epilogue), which leaves 2 such workgroups per CU, so it runs 256
 // threads per part (5 per CU): batched solves 310 -> 323 per second.
+// The autotuner may set a plan's fused-stage kernel to 512 threads per part
+// (pt_threads), which halves each thread's edges at the cost of registers.
 template <typename Real>
 static int launch_part(bool coloured, const InvPlan& P, const Real* w, size_t lds, op_dat var, op_dat flux,
                        const int* status, int fresh, const PartStage* st = nullptr)
 {
-    if constexpr (sizeof(Real) == 4) {
-        if (st) return launch_part_n<Real, 256>(coloured, P, w, lds, var, flux, status, fresh, st);
-        return launch_part_n<Real, 512>(coloured, P, w, lds, var, flux, status, fresh, st);
-    } else {
+    if (st) {
+        if (P.pt_threads == 512) return launch_part_n<Real, 512>(coloured, P, w, lds, var, flux, status, fresh, st);
         return launch_part_n<Real, 256>(coloured, P, w, lds, var, flux, status, fresh, st);
     }
+    if constexpr (sizeof(Real) == 4) return launch_part_n<Real, 512>(coloured, P, w, lds, var, flux, status, fresh, st);
+    return launch_part_n<Real, 256>(coloured, P, w, lds, var, flux, status, fresh, st);
 }
 
 // Plans and packed weights a flux launch needs; allocates and uploads, so it
@@ -1686,6 +1691,26 @@ int mgcfd_plan_stats(op_map map, int* stats)
     return 0;
 }
 
+int mgcfd_map_part_config(op_map map, int part_nodes, int part_threads, int record_bytes)
+{
+    if (!R.inited) return fail("mgcfd_map_part_config: op_init has not been called");
+    if (!map || map->dim != 2) return fail("mgcfd_map_part_config: needs an edge map of dimension 2");
+    if (part_threads != 0 && part_threads != 256 && part_threads != 512)
+        return fail("mgcfd_map_part_config: %d threads per part (0, 256 or 512)", part_threads);
+    if (record_bytes != 0 && record_bytes != 40 && record_bytes != 48)
+        return fail("mgcfd_map_part_config: %d-byte records (0, 40 or 48)", record_bytes);
+    TRY(ensure_flux_plan(map, MGCFD_FLUX_PART_ATOMIC));
+    InvPlan& P = M(map)->inv;
+    const int rs = record_bytes / 8;
+    if (P.pt_rs_set != rs) {
+        P.pt_rs_set = rs;
+        P.pt_psize = -1;                 // re-plan below even at the same part size
+    }
+    TRY(set_part_nodes(map, part_nodes));
+    P.pt_threads = part_threads;
+    return 0;
+}
+
 int mgcfd_part_plan_stats(op_map map, int* stats)
 {
     if (!map || !stats) return fail("mgcfd_part_plan_stats: null argument");
@@ -1856,7 +1881,7 @@ int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_
     TRY(direct(a2, set, 1, false, L, "arg2 dt[1] OP_WRITE"));
     if (set->size)
         k_calc_dt<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
-                                                            nullptr, nullptr, nullptr, MinPub{});
+                                                            nullptr, nullptr, MinPub{});
     touch(a2.dat);
     return check_launch();
 }

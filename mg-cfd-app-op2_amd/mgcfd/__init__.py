@@ -355,6 +355,10 @@ class Runtime:
     def partition(self, nodes=None, coords=None):
         _check(self.L.op_partition(b"PARMETIS", b"KWAY", nodes, None, coords), "op_partition")
 
+    def map_part_config(self, m, part_nodes: int, part_threads: int = 0, record_bytes: int = 0):
+        """Part plan configuration of an edge map (include/mgcfd_op2.h, mgcfd_map_part_config)."""
+        _check(self.L.mgcfd_map_part_config(m, part_nodes, part_threads, record_bytes), "mgcfd_map_part_config")
+
     def set_flux_variant(self, v: int):
         _check(self.L.mgcfd_set_flux_variant(v), "mgcfd_set_flux_variant")
 
@@ -385,10 +389,16 @@ class Batch:
         _check(self.L.mgcfd_batch_create(self.n, arr, ma, aa, FLUX_VARIANTS[variant], C.byref(self.h)),
                "mgcfd_batch_create")
 
-    def autotune(self, iters: int = 5) -> int:
-        p = C.c_int(0)
-        _check(self.L.mgcfd_batch_autotune(self.h, iters, C.byref(p)), "mgcfd_batch_autotune")
-        return p.value
+    def autotune(self, iters: int = 5) -> dict:
+        """Per level, the nodes per part and threads per part the autotuner chose."""
+        L = len(self.sizes[0])
+        nodes, threads = (C.c_int * L)(), (C.c_int * L)()
+        _check(self.L.mgcfd_batch_autotune_levels(self.h, iters, nodes, threads), "mgcfd_batch_autotune_levels")
+        return dict(part_nodes=list(nodes), part_threads=list(threads))
+
+    def set_part_config(self, level: int, part_nodes: int, part_threads: int):
+        _check(self.L.mgcfd_batch_set_part_config(self.h, level, part_nodes, part_threads),
+               "mgcfd_batch_set_part_config")
 
     def run(self, cycles: int, use_graph: bool = True):
         rms = np.zeros((self.n, max(cycles, 1)))

@@ -79,14 +79,35 @@ def test_batch_result_independent_of_mix(rt, meshes, variant):
     mix.close()
 
 
-def test_batch_autotune_keeps_results(rt, meshes, oracle_solves):
-    b = mgcfd.Batch(rt, meshes, MACH, AOA, "part_coloured")
-    p = b.autotune(3)
-    assert p in (256, 128, 64)
+@pytest.mark.parametrize("variant,tol", [("part_coloured", STATE_TOL), ("mixed_atomic", MIXED_STATE_TOL)])
+def test_batch_autotune_keeps_results(rt, meshes, oracle_solves, variant, tol):
+    """Tuning runs trial cycles on the batch and restores its state: the run
+    after it matches the oracle from the setup state."""
+    b = mgcfd.Batch(rt, meshes, MACH, AOA, variant)
+    p = b.autotune(2)
+    print(f"{variant}: autotuned {p}")
+    assert all(x in (256, 192, 128) for x in p["part_nodes"])
+    assert all(x in (256, 512) for x in p["part_threads"])
     st, _ = b.run(3)
     assert (st == 0).all()
     for i in range(len(meshes)):
-        assert rel(b.fetch(i, 0), oracle_solves[i]["levels"][0]["variables"]) <= STATE_TOL
+        for l in range(3):
+            assert rel(b.fetch(i, l), oracle_solves[i]["levels"][l]["variables"]) <= tol, (i, l)
+    b.close()
+
+
+@pytest.mark.parametrize("nodes,threads", [(256, 512), (192, 256), (128, 512)])
+@pytest.mark.parametrize("variant,tol", [("part_atomic", STATE_TOL), ("mixed_coloured", MIXED_STATE_TOL)])
+def test_batch_part_configs_match_oracle(rt, meshes, oracle_solves, nodes, threads, variant, tol):
+    """Every part configuration the autotuner may choose, on every level."""
+    b = mgcfd.Batch(rt, meshes, MACH, AOA, variant)
+    for l in range(3):
+        b.set_part_config(l, nodes, threads)
+    st, _ = b.run(3)
+    assert (st == 0).all()
+    for i in range(len(meshes)):
+        for l in range(3):
+            assert rel(b.fetch(i, l), oracle_solves[i]["levels"][l]["variables"]) <= tol, (i, l)
     b.close()
 
 

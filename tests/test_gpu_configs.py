@@ -249,7 +249,7 @@ def test_batch64_each_mesh_vs_its_own_solve(rt, batch64, variant):
 def test_batch64_smallest_meshes_vs_oracle(rt, orc, batch64):
     meshes, mach, aoa = batch64
     alone = batch64_alone(rt, batch64)
-    order = np.argsort([len(m[0]["coords"]) for m in meshes])[:2]
+    order = [int(i) for i in np.argsort([len(m[0]["coords"]) for m in meshes])[:2]]
     b = mgcfd.Batch(rt, meshes, mach, aoa, "mixed_coloured")
     b.run(30)
     for i in order:
