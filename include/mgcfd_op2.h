@@ -151,6 +151,18 @@ int op_par_loop_copy_double_kernel(const char* name, op_set set, op_arg a0, op_a
 int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op_arg a2);
 /* time_stepping_kernels.h:34-41, site euler3d.cpp:477-479 (gbl OP_MIN) */
 int op_par_loop_get_min_dt_kernel(const char* name, op_set set, op_arg a0, op_arg a1);
+/* get_min_dt_kernel without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and result as op_par_loop_get_min_dt_kernel, but
+ * the per-workgroup minima live in a fixed device array of the library
+ * (65536 workgroups: sets of at most 16,777,216 elements; larger sets are
+ * refused with an error) instead of a buffer the library grows with
+ * hipMalloc.  It allocates nothing, so it refuses to run while sets, maps
+ * or dats declared since the last loop are pending (the next loop's
+ * renumbering allocates).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU race. */
+int mgcfd_get_min_dt_noalloc(const char* name, op_set set, op_arg a0, op_arg a1);
 /* time_stepping_kernels.h:43-64, site euler3d.cpp:485-489 */
 int op_par_loop_compute_step_factor_kernel(const char* name, op_set set, op_arg a0, op_arg a1,
                                            op_arg a2, op_arg a3);

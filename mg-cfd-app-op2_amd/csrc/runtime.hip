@@ -167,6 +167,17 @@ struct Runtime {
 
 static Runtime R;
 
+// Scratch of mgcfd_get_min_dt_noalloc, the entry point for callers that
+// cannot allocate device memory: per-workgroup minima in [0, kNoAllocBlocks)
+// and the result in [kNoAllocBlocks].  A file-scope device array, so every
+// GPU the library runs on holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress gives on each
+// device (scratch_dev), never through the array's name.
+constexpr int kNoAllocBlocks = 65536;           // sets of up to 65536 * 256 = 16,777,216 elements
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocBlocks + 1];
+static double* scratch_dev[kMaxDevices];        // device address of 'scratch' on each device (op_init)
+
 static SetImpl* S(op_set s) { return static_cast<SetImpl*>(s->impl); }
 static MapImpl* M(op_map m) { return static_cast<MapImpl*>(m->impl); }
 static DatImpl* D(op_dat d) { return static_cast<DatImpl*>(d->impl); }
@@ -1413,6 +1424,15 @@ int op_init(int argc, char** argv, int diags)
     if (ndev <= 0) return fail("op_init: no HIP device visible");
     if (R.device >= ndev) return fail("op_init: device %d not present (%d visible)", R.device, ndev);
     HIPCHK(hipSetDevice(R.device));
+    if (R.device >= kMaxDevices) return fail("op_init: device %d is beyond the %d this library serves", R.device,
+                                             kMaxDevices);
+    if (!scratch_dev[R.device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail("op_init: no device address for the no-allocation scratch: %s", hipGetErrorString(e));
+        scratch_dev[R.device] = static_cast<double*>(p);
+    }
     {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, R.device));
@@ -1904,6 +1924,35 @@ int op_par_loop_get_min_dt_kernel(const char* name, op_set set, op_arg a0, op_ar
     TRY(check_launch());
     double m;
     TRY(read_scalar(R.d_scal, &m));
+    std::memcpy(a1.data, &m, sizeof(double));
+    return 0;
+}
+
+int mgcfd_get_min_dt_noalloc(const char* name, op_set set, op_arg a0, op_arg a1)
+{
+    const char* L = "mgcfd_get_min_dt_noalloc";
+    (void)name;
+    if (!R.inited) return fail("%s: op_init has not been called", L);
+    // the first loop after a declaration renumbers and uploads (it allocates)
+    if (!R.frozen || R.dirty)
+        return fail("%s: sets, maps or dats were declared since the last loop; run any op_par_loop first", L);
+    TRY(direct(a0, set, 1, false, L, "arg0 dt[1] OP_READ"));
+    TRY(gbl(a1, "double", OP_MIN, L, "arg1 gbl double OP_MIN"));
+    const int n = set->size;
+    double init;
+    std::memcpy(&init, a1.data, sizeof(double));
+    if (n == 0) return 0;
+    const long long nb = nblk(n);
+    if (nb > kNoAllocBlocks)
+        return fail("%s: the set has %d elements, more than the %lld its fixed scratch serves", L, n,
+                    (long long)kNoAllocBlocks * kBlock);
+    double* ptr = scratch_dev[R.device];
+    if (!ptr) return fail("%s: no scratch address for device %d", L, R.device);
+    k_min_part<<<(unsigned)nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), ptr);
+    k_min_final<<<1, kBlock, 0, R.stream>>>((int)nb, ptr, init, ptr + kNoAllocBlocks, nullptr);
+    TRY(check_launch());
+    double m;
+    TRY(read_scalar(ptr + kNoAllocBlocks, &m));
     std::memcpy(a1.data, &m, sizeof(double));
     return 0;
 }
