@@ -349,17 +349,21 @@ int mgcfd_batch_create(int n, const mgcfd_mesh* meshes, const double* mach, cons
                        int flux_variant, mgcfd_batch* out);
 /* Autotuner of the part variants' kernels (the edge flux with its fused
  * Runge-Kutta stage, which every cycle runs): per level, the nodes per part
- * (256, 192 or 128) and threads per part (256 or 512) that give the fastest
- * captured multigrid cycle of the whole batch ('iters' timed cycles per
- * candidate, levels tuned in turn).  The batch's state is left as it was.
- * part_nodes[levels] / part_threads[levels] (nullable) receive the choice;
- * variants without a part plan have nothing to tune (0s).
- * mgcfd_batch_autotune is the same tuning, reporting level 0's part size. */
+ * (256, 192 or 128), threads per part (256 or 512) and launch form (one
+ * workgroup per part, or 2 or 4 persistent workgroups per CU walking their
+ * XCD's parts) that give the fastest captured multigrid cycle of the whole
+ * batch ('iters' timed cycles per candidate, levels tuned in turn).  The
+ * batch's state is left as it was.  part_nodes / part_threads /
+ * persistent_per_cu [levels] (nullable) receive the choice; variants
+ * without a part plan have nothing to tune (0s).  mgcfd_batch_autotune is
+ * the same tuning, reporting level 0's part size. */
 int mgcfd_batch_autotune(mgcfd_batch b, int iters, int* part_nodes);
-int mgcfd_batch_autotune_levels(mgcfd_batch b, int iters, int* part_nodes, int* part_threads);
+int mgcfd_batch_autotune_levels(mgcfd_batch b, int iters, int* part_nodes, int* part_threads,
+                                int* persistent_per_cu);
 /* Sets one level's part configuration directly (a tuning found earlier):
- * part_nodes in [16, 256], part_threads 256 or 512.  Re-plans the level. */
-int mgcfd_batch_set_part_config(mgcfd_batch b, int level, int part_nodes, int part_threads);
+ * part_nodes in [16, 256], part_threads 256 or 512, persistent_per_cu 0
+ * (one workgroup per part) or 1..8.  Re-plans the level. */
+int mgcfd_batch_set_part_config(mgcfd_batch b, int level, int part_nodes, int part_threads, int persistent_per_cu);
 /* Runs 'cycles' multigrid cycles from the setup state; rms receives
  * [n][cycles] L0 RMS residuals and status [n] (0 ok, 1 negative min dt,
  * 2 NaN/Inf; either may be NULL). */

@@ -1237,10 +1237,20 @@ struct StageArgs {
 template <int NT>
 __device__ __forceinline__ double stage_min_dt(const StageArgs& a, int seg, bool first)
 {
-    static_assert(NT >= kMinSlots, "one slot per thread");
-    const unsigned long long k =
-        threadIdx.x < kMinSlots ? a.mslots[(size_t)seg * kMinSlots + threadIdx.x] : kMinEmpty;
-    const double m = block_min_all<NT>(min_unkey(k));
+    // the slot minimum over the images (unsigned order), with uniform loads:
+    // the reduction stays in scalar registers, off the vector registers the
+    // edge phase needs
+    const unsigned* sl = reinterpret_cast<const unsigned*>(a.mslots + (size_t)seg * kMinSlots);
+    unsigned khi = (unsigned)(kMinEmpty >> 32), klo = (unsigned)kMinEmpty;
+#pragma unroll 8
+    for (int j = 0; j < kMinSlots; j++) {
+        const unsigned yhi = __builtin_amdgcn_readfirstlane(sl[2 * j + 1]);
+        const unsigned ylo = __builtin_amdgcn_readfirstlane(sl[2 * j]);
+        const bool lt = yhi < khi || (yhi == khi && ylo < klo);
+        khi = lt ? yhi : khi;
+        klo = lt ? ylo : klo;
+    }
+    const double m = min_unkey(((unsigned long long)khi << 32) | klo);
     const double mdt = (m < DBL_MAX) ? m : DBL_MAX;
     if (first && threadIdx.x == 0) {
         if (mdt < 0.0 && a.mstatus[seg] == 0) a.mstatus[seg] = 1;
@@ -1395,25 +1405,17 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 // different buffers.  The edge flux starts from the flux dat on the first
 // stage (fresh == 0) and from zero after it; it is not written back.
 // pseg / pfirst: segment of each part and first part of each segment
-// (batched solves), null for one mesh.
+// (batched solves), null for one mesh.  Returns false when the context
+// aborts (negative minimum dt).
 template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
-__global__ __launch_bounds__(T) void k_flux_part_stage(
-    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+__device__ __forceinline__ bool part_stage_one(
+    int p, const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
-    StageArgs a, Consts k0, const int* __restrict__ pseg, const int* __restrict__ pfirst)
+    const StageArgs& a, const Consts& k0, const int* __restrict__ pseg, const int* __restrict__ pfirst, double* sm)
 {
-    extern __shared__ double sm[];
-    if (halted(a.status)) return;
-    stage_reset_min<T>(a);
-    const int p = xcd_block();
     const PartInfo P = parts[p];
     const int seg = pseg ? pseg[p] : 0;
     static_assert(T >= kPart, "one owned node per thread");
-    double mdt = 0.0;
-    if (a.rk == 0) {
-        mdt = stage_min_dt<T>(a, seg, pseg ? p == pfirst[seg] : p == 0);
-        if (mdt < 0.0 && a.status) return;
-    }
     // the owned node's old state and volume / step factor, in flight through
     // the edge phase (clamped to a valid node for threads without one)
     const int t = threadIdx.x;
@@ -1424,6 +1426,13 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
     const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
     double* acc = part_accumulate<R, T, EPT, NPT, COLOURED, RS, true>(P, halo, ecode, ew, nloc, rec_cap, vin,
                                                                       a.flux, k0, fresh, sm);
+    // the minimum dt after the edge phase (which writes only LDS), before any
+    // update: fewer live registers through the edge phase
+    double mdt = 0.0;
+    if (a.rk == 0) {
+        mdt = stage_min_dt<T>(a, seg, pseg ? p == pfirst[seg] : p == 0);
+        if (mdt < 0.0 && a.status) return false;
+    }
     double s = 0.0;
     int nb = 0;
     if (t < P.cnt) {
@@ -1433,6 +1442,47 @@ __global__ __launch_bounds__(T) void k_flux_part_stage(
         stage_node(a, k0, seg, i, vin, f, mdt, o, vs, q0, q1, s, nb);
     }
     stage_sums<T>(a, p, s, nb);
+    return true;
+}
+
+// One workgroup per part.
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
+__global__ __launch_bounds__(T) void k_flux_part_stage(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
+    StageArgs a, Consts k0, const int* __restrict__ pseg, const int* __restrict__ pfirst)
+{
+    extern __shared__ double sm[];
+    if (halted(a.status)) return;
+    if (part_stage_one<R, T, EPT, NPT, COLOURED, RS>(xcd_block(), parts, halo, ecode, ew, nloc, rec_cap, vin, fresh,
+                                                     a, k0, pseg, pfirst, sm))
+        stage_reset_min<T>(a);
+}
+
+// Persistent workgroups (a grid of a multiple of 8, below nparts): the
+// workgroups of each XCD walk that XCD's contiguous slice of the parts with
+// a stride, so the slice's halo stays in the XCD's L2 as it does with one
+// workgroup per part.  A kernel of its own: the loop costs the one-part
+// kernel ~30 VGPRs, a wave per SIMD.
+template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
+__global__ __launch_bounds__(T) void k_flux_part_stage_persistent(
+    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
+    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
+    StageArgs a, Consts k0, const int* __restrict__ pseg, const int* __restrict__ pfirst, int nparts)
+{
+    extern __shared__ double sm[];
+    if (halted(a.status)) return;
+    stage_reset_min<T>(a);
+    const int x = blockIdx.x % 8, k = blockIdx.x / 8, g = gridDim.x / 8;
+    const int per = nparts / 8, rem = nparts % 8;
+    const int lo = (x < rem) ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    const int hi = lo + per + (x < rem ? 1 : 0);
+    for (int p = lo + k; p < hi; p += g) {
+        if (!part_stage_one<R, T, EPT, NPT, COLOURED, RS>(p, parts, halo, ecode, ew, nloc, rec_cap, vin, fresh, a,
+                                                          k0, pseg, pfirst, sm))
+            return;
+        __syncthreads();        // the next part reuses the LDS
+    }
 }
 
 // residual_kernel (validation.h:27-35); with 'part' set also the per-node

@@ -118,6 +118,8 @@ struct InvPlan {
     size_t pt_lds32 = 0;
     int pt_psize = kPart;            // max nodes per part (autotuned)
     int pt_threads = 0;              // threads per part of the kernel with the fused stage (0: default; autotuned)
+    int pt_persist = 0;              // fused-stage kernel: 0 one workgroup per part, else persistent workgroups
+                                     // per CU (autotuned)
     int pt_rs_set = 0;               // FP64 record values of the edge-flux kernel: 0 by the rule below, or 5 / 6
     bool bk_built = false, pt_built = false;
     std::vector<int> h_ptr;          // host CSR kept for re-planning (dim-2 dense maps)
@@ -554,15 +556,22 @@ static int freeze()
         if (s.impl) S(&s)->frozen = true;
     R.frozen = true;
     R.dirty = false;
-    for (auto& m : R.maps) {
-        if (!m.impl || M(&m)->on_device) continue;
-        if (int rc = build_map_device(&m)) return rc;
-        M(&m)->on_device = true;
-    }
+    // maps and dats permuted into the internal order and uploaded, each on
+    // a host thread of its own (HIP calls are thread-safe; each touches only
+    // its own object)
+    std::vector<op_map> ml;
+    for (auto& m : R.maps)
+        if (m.impl && !M(&m)->on_device) ml.push_back(&m);
+    if (int rc = run_tasks((int)ml.size(), [&](int k) -> int {
+        if (int r = build_map_device(ml[k])) return r;
+        M(ml[k])->on_device = true;
+        return 0;
+    }))
+        return rc;
+    std::vector<op_dat> dl;
     for (auto& d : R.dats)
-        if (d.impl)
-            if (int rc = upload_dat(&d)) return rc;
-    return 0;
+        if (d.impl && (!D(&d)->d || !D(&d)->pending.empty())) dl.push_back(&d);
+    return run_tasks((int)dl.size(), [&](int k) -> int { return upload_dat(dl[k]); });
 }
 
 // Uploads a dat's pending AoS host data (or zeros) into its device buffer
@@ -750,6 +759,63 @@ struct PartOut {
     std::string err;
 };
 
+// LDS bank-aware order of a part's local edges, within each colour (so the
+// coloured kernel's colour phases, and with them its sums, are unchanged).
+// The kernels give consecutive local edges to consecutive lanes, and the LDS
+// serves a wave's lanes in groups: 8 lanes per pass for the 16-byte record
+// reads of 48-byte records (whose bank is fixed by the slot modulo 8), 16
+// per pass for 8-byte accesses (accumulators at a 40-byte stride and
+// 40-byte records: slot modulo 16).  Each position takes, among the next
+// candidates of its colour, the edge whose endpoint slots collide least
+// with those already in its 8- and 16-lane groups.
+template <typename LEv, typename SlotFn>
+static void bank_order(std::vector<LEv>& le, SlotFn slot)
+{
+    constexpr size_t kLook = 48;
+    std::vector<LEv> out;
+    out.reserve(le.size());
+    std::vector<int> sa(le.size()), sb(le.size());
+    for (size_t i = 0; i < le.size(); i++) {
+        sa[i] = slot(le[i].a);
+        sb[i] = slot(le[i].b);
+    }
+    std::vector<size_t> rest;
+    unsigned m8a = 0, m8b = 0, m16a = 0, m16b = 0;
+    for (size_t lo = 0; lo < le.size();) {
+        size_t hi = lo;
+        while (hi < le.size() && le[hi].col == le[lo].col) hi++;
+        rest.clear();
+        for (size_t i = lo; i < hi; i++) rest.push_back(i);
+        size_t head = 0;
+        while (head < rest.size()) {
+            const size_t pos = out.size();
+            if (pos % 16 == 0) m16a = m16b = 0;
+            if (pos % 8 == 0) m8a = m8b = 0;
+            size_t best = head;
+            int bc = 1 << 30;
+            for (size_t k = head; k < rest.size() && k < head + kLook; k++) {
+                const size_t i = rest[k];
+                const int c = ((m8a >> (sa[i] & 7)) & 1) + ((m8b >> (sb[i] & 7)) & 1) +
+                              ((m16a >> (sa[i] & 15)) & 1) + ((m16b >> (sb[i] & 15)) & 1);
+                if (c < bc) {
+                    bc = c;
+                    best = k;
+                    if (c == 0) break;
+                }
+            }
+            std::swap(rest[head], rest[best]);
+            const size_t i = rest[head++];
+            m8a |= 1u << (sa[i] & 7);
+            m8b |= 1u << (sb[i] & 7);
+            m16a |= 1u << (sa[i] & 15);
+            m16b |= 1u << (sb[i] & 15);
+            out.push_back(le[i]);
+        }
+        lo = hi;
+    }
+    le.swap(out);
+}
+
 static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, const std::vector<int2>& inc2,
                       PartOut& out)
 {
@@ -816,6 +882,7 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
             ncol = std::max(ncol, c + 1);
         }
         std::stable_sort(le.begin(), le.end(), [](const LE& x, const LE& y) { return x.col < y.col; });
+        bank_order(le, [&](int nd) { return slot(nd); });
         PartInfo pi;
         pi.n0 = lo;
         pi.cnt = hi - lo;
@@ -1222,6 +1289,18 @@ static int launch_part_rs(bool coloured, const InvPlan& P, const Real* w, size_t
                           const int* status, int fresh, const PartStage* st)
 {
     if (st) {
+        const int pw = 8 * ((R.n_cu * P.pt_persist + 7) / 8);    // persistent grid: a multiple of the 8 XCDs
+        if (P.pt_persist > 0 && pw < P.pt_parts) {
+            if (coloured)
+                k_flux_part_stage_persistent<Real, T, EPT, NPT, true, RS><<<pw, T, lds, R.stream>>>(
+                    P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
+                    st->pseg, st->pfirst, P.pt_parts);
+            else
+                k_flux_part_stage_persistent<Real, T, EPT, NPT, false, RS><<<pw, T, lds, R.stream>>>(
+                    P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
+                    st->pseg, st->pfirst, P.pt_parts);
+            return check_launch();
+        }
         if (coloured)
             k_flux_part_stage<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, R.stream>>>(
                 P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,

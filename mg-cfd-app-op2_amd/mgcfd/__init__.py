@@ -390,14 +390,16 @@ class Batch:
                "mgcfd_batch_create")
 
     def autotune(self, iters: int = 5) -> dict:
-        """Per level, the nodes per part and threads per part the autotuner chose."""
+        """Per level, the nodes per part, threads per part and persistent
+        workgroups per CU (0: one workgroup per part) the autotuner chose."""
         L = len(self.sizes[0])
-        nodes, threads = (C.c_int * L)(), (C.c_int * L)()
-        _check(self.L.mgcfd_batch_autotune_levels(self.h, iters, nodes, threads), "mgcfd_batch_autotune_levels")
-        return dict(part_nodes=list(nodes), part_threads=list(threads))
+        nodes, threads, persist = (C.c_int * L)(), (C.c_int * L)(), (C.c_int * L)()
+        _check(self.L.mgcfd_batch_autotune_levels(self.h, iters, nodes, threads, persist),
+               "mgcfd_batch_autotune_levels")
+        return dict(part_nodes=list(nodes), part_threads=list(threads), persistent_per_cu=list(persist))
 
-    def set_part_config(self, level: int, part_nodes: int, part_threads: int):
-        _check(self.L.mgcfd_batch_set_part_config(self.h, level, part_nodes, part_threads),
+    def set_part_config(self, level: int, part_nodes: int, part_threads: int, persistent_per_cu: int = 0):
+        _check(self.L.mgcfd_batch_set_part_config(self.h, level, part_nodes, part_threads, persistent_per_cu),
                "mgcfd_batch_set_part_config")
 
     def run(self, cycles: int, use_graph: bool = True):

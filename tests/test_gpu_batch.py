@@ -88,6 +88,7 @@ def test_batch_autotune_keeps_results(rt, meshes, oracle_solves, variant, tol):
     print(f"{variant}: autotuned {p}")
     assert all(x in (256, 192, 128) for x in p["part_nodes"])
     assert all(x in (256, 512) for x in p["part_threads"])
+    assert all(x in (0, 2, 4) for x in p["persistent_per_cu"])
     st, _ = b.run(3)
     assert (st == 0).all()
     for i in range(len(meshes)):
@@ -96,13 +97,13 @@ def test_batch_autotune_keeps_results(rt, meshes, oracle_solves, variant, tol):
     b.close()
 
 
-@pytest.mark.parametrize("nodes,threads", [(256, 512), (192, 256), (128, 512)])
+@pytest.mark.parametrize("nodes,threads,persist", [(256, 512, 0), (192, 256, 0), (128, 512, 0), (256, 256, 4)])
 @pytest.mark.parametrize("variant,tol", [("part_atomic", STATE_TOL), ("mixed_coloured", MIXED_STATE_TOL)])
-def test_batch_part_configs_match_oracle(rt, meshes, oracle_solves, nodes, threads, variant, tol):
+def test_batch_part_configs_match_oracle(rt, meshes, oracle_solves, nodes, threads, persist, variant, tol):
     """Every part configuration the autotuner may choose, on every level."""
     b = mgcfd.Batch(rt, meshes, MACH, AOA, variant)
     for l in range(3):
-        b.set_part_config(l, nodes, threads)
+        b.set_part_config(l, nodes, threads, persist)
     st, _ = b.run(3)
     assert (st == 0).all()
     for i in range(len(meshes)):
@@ -120,3 +121,22 @@ def test_batch_graph_and_eager_agree(rt, meshes):
         assert np.array_equal(b1.fetch(i, 0), b2.fetch(i, 0))
     b1.close()
     b2.close()
+
+
+@pytest.mark.parametrize("variant,tol", [("part_coloured", STATE_TOL), ("mixed_atomic", MIXED_STATE_TOL)])
+def test_batch_persistent_workgroups_walk_many_parts(rt, variant, tol):
+    """Persistent workgroups (one per CU) with 64-node parts on a 40K-node
+    mesh: every workgroup walks several parts of its XCD's slice."""
+    ms = [mgcfd.mesh_knn(40000, 124000, 3, seed=21), mgcfd.mesh_knn(3000, 9300, 3, seed=22)]
+    mach, aoa = [0.5, 0.6], [3.0, 1.0]
+    o = Oracle("orc")
+    refs = [o.solve(m, 3, mach[i], aoa[i]) for i, m in enumerate(ms)]
+    b = mgcfd.Batch(rt, ms, mach, aoa, variant)
+    for l in range(3):
+        b.set_part_config(l, 64, 256, 1)
+    st, _ = b.run(3)
+    assert (st == 0).all()
+    for i in range(2):
+        for l in range(3):
+            assert rel(b.fetch(i, l), refs[i]["levels"][l]["variables"]) <= tol, (i, l)
+    b.close()
