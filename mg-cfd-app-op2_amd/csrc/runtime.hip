@@ -23,7 +23,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <numeric>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -148,7 +150,7 @@ struct Runtime {
     bool inited = false;
     int device = 0;
     int n_cu = 256;              // compute units of the device
-    hipStream_t stream = nullptr;
+    hipStream_t lib_stream = nullptr;   // the library's own stream (op_init)
     int base_index = 0;
     bool frozen = false;         // at least one loop has run
     bool dirty = false;          // objects declared since the last freeze
@@ -168,6 +170,18 @@ struct Runtime {
 };
 
 static Runtime R;
+
+// Serialises the runtime's registries (sets, maps, dats, names) and the
+// ordering and upload of newly declared objects: served requests are built
+// on worker threads while the request loop drives the running ones.
+static std::recursive_mutex g_rt_mu;
+using RtLock = std::lock_guard<std::recursive_mutex>;
+
+// The stream the launchers enqueue on: the library's own, unless the calling
+// thread has set another (StreamScope: a batch or served request runs its
+// work on its context's stream, on whichever host thread builds or drives it).
+static thread_local hipStream_t tl_stream = nullptr;
+static inline hipStream_t cur_stream() { return tl_stream ? tl_stream : R.lib_stream; }
 
 // Scratch of mgcfd_get_min_dt_noalloc, the entry point for callers that
 // cannot allocate device memory: per-workgroup minima in [0, kNoAllocBlocks)
@@ -359,6 +373,7 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
 
 static int upload_dat(op_dat d);
 static void mg_release_all();
+static void mg_join_admissions();
 
 // Node coordinates [n][3] of a set not yet ordered: its internal order will
 // be geometric.
@@ -449,8 +464,12 @@ static int run_tasks(int n, F&& f)
     std::atomic<int> next{0};
     std::vector<std::thread> pool;
     const int nth = std::min(n, 16);
+    const hipStream_t parent = tl_stream;
     for (int t = 0; t < nth; t++)
         pool.emplace_back([&] {
+            // tasks may allocate, copy and launch: the caller's device and stream
+            hipSetDevice(R.device);
+            tl_stream = parent;
             for (int i; (i = next.fetch_add(1)) < n;) {
                 rc[i] = f(i);
                 if (rc[i]) msg[i] = g_err;       // the worker's own message
@@ -467,6 +486,7 @@ static int run_tasks(int n, F&& f)
 
 static int freeze()
 {
+    RtLock lk(g_rt_mu);
     if (R.frozen && !R.dirty) return 0;
     // node-like sets: targets of dim-2 maps (each set on its own thread)
     struct NodeSet {
@@ -1082,7 +1102,7 @@ static int set_part_nodes(op_map m, int nodes)
     InvPlan& P = M(m)->inv;
     if (!P.dense || P.h_ptr.empty()) return fail("set_part_nodes: map %s has no part plan", m->name);
     if (P.pt_psize == nodes && P.pt_built) return 0;
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     free_partplan(P);
     P.pt_built = false;
     P.pt_psize = nodes;
@@ -1106,6 +1126,7 @@ static void free_invplan(InvPlan& P)
 // served solves that end); the handle stays valid as an empty, dead object.
 static void release_dat(op_dat d)
 {
+    RtLock lk(g_rt_mu);
     if (!d || !d->impl) return;
     DatImpl* di = D(d);
     for (void* p : {(void*)di->d, (void*)di->d_pack, (void*)di->d_tmp})
@@ -1116,6 +1137,7 @@ static void release_dat(op_dat d)
 
 static void release_map(op_map m)
 {
+    RtLock lk(g_rt_mu);
     if (!m || !m->impl) return;
     MapImpl* mi = M(m);
     if (mi->d_map) hipFree(mi->d_map);
@@ -1126,6 +1148,7 @@ static void release_map(op_map m)
 
 static void release_set(op_set s)
 {
+    RtLock lk(g_rt_mu);
     if (!s || !s->impl) return;
     delete S(s);
     s->impl = nullptr;
@@ -1160,7 +1183,7 @@ static int packed_weights(op_dat w, const double4** out)
     if (!di->d_pack) HIPCHK(hipMalloc(&di->d_pack, sizeof(double4) * std::max(1, n)));
     if (di->pack_version != di->version) {
         if (n > 0) {
-            k_pack_weights<<<nblk(n), kBlock, 0, R.stream>>>(n, (const double*)di->d, di->d_pack);
+            k_pack_weights<<<nblk(n), kBlock, 0, cur_stream()>>>(n, (const double*)di->d, di->d_pack);
             if (int rc = check_launch()) return rc;
         }
         di->pack_version = di->version;
@@ -1213,16 +1236,16 @@ static int gbl(const op_arg& a, const char* type, op_access acc, const char* loo
 // Host readback of a device scalar, synchronising the stream.
 static int read_scalar(const double* d_src, double* out)
 {
-    HIPCHK(hipMemcpyAsync(R.h_pin, d_src, sizeof(double), hipMemcpyDeviceToHost, R.stream));
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipMemcpyAsync(R.h_pin, d_src, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     *out = *R.h_pin;
     return 0;
 }
 
 static int read_iscalar(const int* d_src, int* out)
 {
-    HIPCHK(hipMemcpyAsync(R.h_pini, d_src, sizeof(int), hipMemcpyDeviceToHost, R.stream));
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipMemcpyAsync(R.h_pini, d_src, sizeof(int), hipMemcpyDeviceToHost, cur_stream()));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     *out = *R.h_pini;
     return 0;
 }
@@ -1237,7 +1260,7 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
 {
     if ((variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST) && P.bk_blocks > 0 &&
         (P.bk_wdat != (const void*)w || P.bk_wver != D(w)->version)) {
-        k_pack_incidence_weights3<<<nblk(P.bk_entries), kBlock, 0, R.stream>>>(P.bk_entries, P.d_bk_eid, dptr(w),
+        k_pack_incidence_weights3<<<nblk(P.bk_entries), kBlock, 0, cur_stream()>>>(P.bk_entries, P.d_bk_eid, dptr(w),
                                                                               P.d_bk_w3);
         TRY(check_launch());
         P.bk_wdat = w;
@@ -1246,14 +1269,14 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
     if (mixed_variant(variant) && P.pt_parts > 0 &&
         (!P.d_pt_w32 || P.pt_w32dat != (const void*)w || P.pt_w32ver != D(w)->version)) {
         if (!P.d_pt_w32) HIPCHK(hipMalloc(&P.d_pt_w32, sizeof(float) * 3 * std::max<size_t>(1, (size_t)P.pt_nloc)));
-        k_pack_part_weights32<<<nblk((size_t)P.pt_nloc), kBlock, 0, R.stream>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
+        k_pack_part_weights32<<<nblk((size_t)P.pt_nloc), kBlock, 0, cur_stream()>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
                                                                               P.d_pt_w32);
         TRY(check_launch());
         P.pt_w32dat = w;
         P.pt_w32ver = D(w)->version;
     }
     if (part_variant(variant) && P.pt_parts > 0 && (P.pt_wdat != (const void*)w || P.pt_wver != D(w)->version)) {
-        k_pack_part_weights<<<nblk((size_t)P.pt_nloc), kBlock, 0, R.stream>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
+        k_pack_part_weights<<<nblk((size_t)P.pt_nloc), kBlock, 0, cur_stream()>>>(P.pt_nloc, P.d_pt_eid, dptr(w),
                                                                             P.d_pt_w);
         TRY(check_launch());
         P.pt_wdat = w;
@@ -1265,11 +1288,11 @@ static int refresh_plan_weights(int variant, InvPlan& P, op_dat w)
 static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat flux, const int* status, int fresh)
 {
     if (exact)
-        k_flux_blocked_pf<true, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+        k_flux_blocked_pf<true, 4><<<P.bk_blocks, kBlock, P.bk_lds, cur_stream()>>>(
             P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
             P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     else
-        k_flux_blocked_pf<false, 4><<<P.bk_blocks, kBlock, P.bk_lds, R.stream>>>(
+        k_flux_blocked_pf<false, 4><<<P.bk_blocks, kBlock, P.bk_lds, cur_stream()>>>(
             P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
             P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     return check_launch();
@@ -1292,31 +1315,31 @@ static int launch_part_rs(bool coloured, const InvPlan& P, const Real* w, size_t
         const int pw = 8 * ((R.n_cu * P.pt_persist + 7) / 8);    // persistent grid: a multiple of the 8 XCDs
         if (P.pt_persist > 0 && pw < P.pt_parts) {
             if (coloured)
-                k_flux_part_stage_persistent<Real, T, EPT, NPT, true, RS><<<pw, T, lds, R.stream>>>(
+                k_flux_part_stage_persistent<Real, T, EPT, NPT, true, RS><<<pw, T, lds, cur_stream()>>>(
                     P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                     st->pseg, st->pfirst, P.pt_parts);
             else
-                k_flux_part_stage_persistent<Real, T, EPT, NPT, false, RS><<<pw, T, lds, R.stream>>>(
+                k_flux_part_stage_persistent<Real, T, EPT, NPT, false, RS><<<pw, T, lds, cur_stream()>>>(
                     P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                     st->pseg, st->pfirst, P.pt_parts);
             return check_launch();
         }
         if (coloured)
-            k_flux_part_stage<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, R.stream>>>(
+            k_flux_part_stage<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
                 P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                 st->pseg, st->pfirst);
         else
-            k_flux_part_stage<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, R.stream>>>(
+            k_flux_part_stage<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
                 P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
                 st->pseg, st->pfirst);
         return check_launch();
     }
     if (coloured)
-        k_flux_part<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     else
-        k_flux_part<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, R.stream>>>(
+        k_flux_part<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
             P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, dptr(var), dptr(flux), R.k, status,
             fresh);
     return check_launch();
@@ -1425,7 +1448,7 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
         TRY(packed_weights(w, &wp));
         const int ne = map->from->size;
         if (ne == 0) return 0;
-        k_flux_atomic<<<nblk(ne), kBlock, 0, R.stream>>>(ne, (const int2*)M(map)->d_map, wp, dptr(var),
+        k_flux_atomic<<<nblk(ne), kBlock, 0, cur_stream()>>>(ne, (const int2*)M(map)->d_map, wp, dptr(var),
                                                          dptr(flux), R.k, status);
         return check_launch();
     }
@@ -1446,10 +1469,10 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     const double4* wp = nullptr;
     TRY(packed_weights(w, &wp));
     if (variant == MGCFD_FLUX_GATHER_FAST)
-        k_flux_gather<false><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
+        k_flux_gather<false><<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(
             P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status, fresh);
     else
-        k_flux_gather<true><<<nblk(P.n_targets), kBlock, 0, R.stream>>>(
+        k_flux_gather<true><<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(
             P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp, dptr(var), dptr(flux), R.k, status, fresh);
     return check_launch();
 }
@@ -1473,7 +1496,7 @@ static int launch_bnd(op_map b2n, op_dat group, op_dat bw, op_dat var, op_dat fl
     TRY(bnd_targets(b2n));
     const InvPlan& P = M(b2n)->inv;
     if (P.n_targets == 0) return 0;
-    k_bnd_flux<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
+    k_bnd_flux<<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
                                                            iptr(group), dptr(bw), dptr(var), dptr(flux), R.k,
                                                            status);
     return check_launch();
@@ -1517,7 +1540,7 @@ int op_init(int argc, char** argv, int diags)
         HIPCHK(hipGetDeviceProperties(&prop, R.device));
         R.n_cu = std::max(1, prop.multiProcessorCount);
     }
-    HIPCHK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&R.lib_stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&R.d_scal, 4 * sizeof(double)));
     HIPCHK(hipMalloc(&R.d_iscal, 4 * sizeof(int)));
     HIPCHK(hipHostMalloc(&R.h_pin, 8 * sizeof(double), hipHostMallocDefault));
@@ -1531,7 +1554,9 @@ int op_init(int argc, char** argv, int diags)
 void op_exit(void)
 {
     if (!R.inited) return;
-    hipStreamSynchronize(R.stream);
+    mg_join_admissions();      // before the lock, which admissions in flight take
+    RtLock lk(g_rt_mu);
+    hipStreamSynchronize(cur_stream());
     mg_release_all();
     for (auto& d : R.dats) release_dat(&d);
     for (auto& m : R.maps) release_map(&m);
@@ -1545,13 +1570,14 @@ void op_exit(void)
     hipFree(R.d_iscal);
     hipHostFree(R.h_pin);
     hipHostFree(R.h_pini);
-    hipStreamDestroy(R.stream);
+    hipStreamDestroy(R.lib_stream);
     R = Runtime();
 }
 
 op_set op_decl_set(int size, const char* name)
 {
     if (size < 0) { fail("op_decl_set %s: negative size", name ? name : ""); return nullptr; }
+    RtLock lk(g_rt_mu);
     R.sets.push_back(op_set_core{});
     op_set s = &R.sets.back();
     s->index = (int)R.sets.size() - 1;
@@ -1593,6 +1619,7 @@ op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char*
 
 static op_map add_map(op_set from, op_set to, int dim, MapImpl* mi, const char* name)
 {
+    RtLock lk(g_rt_mu);
     R.maps.push_back(op_map_core{});
     op_map m = &R.maps.back();
     m->index = (int)R.maps.size() - 1;
@@ -1640,6 +1667,7 @@ static op_dat decl_dat(op_set set, int dim, const char* type, int size, const ch
         for (size_t q = 0; q < (size_t)set->size * dim; q++)
             std::memcpy(di->pending.data() + q * in_esz, data + q * (size_t)size, in_esz);
     }
+    RtLock lk(g_rt_mu);
     R.dats.push_back(op_dat_core{});
     op_dat d = &R.dats.back();
     d->index = (int)R.dats.size() - 1;
@@ -1726,7 +1754,7 @@ int op_fetch_data_char(op_dat d, char* dest)
     const size_t esz = di->is_int ? sizeof(int) : sizeof(double);
     const size_t rec = esz * dim;
     std::vector<char> aos(rec * (size_t)std::max(n, 1));
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     HIPCHK(hipMemcpy(aos.data(), di->d, rec * (size_t)n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) std::memcpy(dest + (size_t)si->to_orig(i) * rec, aos.data() + (size_t)i * rec, rec);
     return 0;
@@ -1739,7 +1767,7 @@ int op_upload_data_char(op_dat d, const char* src)
     DatImpl* di = D(d);
     const size_t esz = di->is_int ? sizeof(int) : sizeof(double);
     di->pending.assign(src, src + (size_t)d->set->size * d->dim * esz);
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     return upload_dat(d);
 }
 
@@ -1765,7 +1793,7 @@ int op_renumber(op_map map)
 
 void op_timers(double* cpu, double* wall)
 {
-    if (R.inited) hipStreamSynchronize(R.stream);
+    if (R.inited) hipStreamSynchronize(cur_stream());
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     if (cpu) *cpu = (double)std::clock() / CLOCKS_PER_SEC;
     if (wall) *wall = t;
@@ -1836,15 +1864,15 @@ int mgcfd_stream_copy(long long bytes, int iters, double* ms_per_call)
         hipFree(a);
         return fail("mgcfd_stream_copy: out of device memory");
     }
-    k_fill<<<nblk(2 * n), kBlock, 0, R.stream>>>(2 * n, reinterpret_cast<double*>(a), 1.0);
+    k_fill<<<nblk(2 * n), kBlock, 0, cur_stream()>>>(2 * n, reinterpret_cast<double*>(a), 1.0);
     const unsigned grid = 256 * 32;
-    k_stream_copy<<<grid, kBlock, 0, R.stream>>>(n, a, b);
+    k_stream_copy<<<grid, kBlock, 0, cur_stream()>>>(n, a, b);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, R.stream));
-    for (int i = 0; i < iters; i++) k_stream_copy<<<grid, kBlock, 0, R.stream>>>(n, a, b);
-    HIPCHK(hipEventRecord(e1, R.stream));
+    HIPCHK(hipEventRecord(e0, cur_stream()));
+    for (int i = 0; i < iters; i++) k_stream_copy<<<grid, kBlock, 0, cur_stream()>>>(n, a, b);
+    HIPCHK(hipEventRecord(e1, cur_stream()));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
@@ -1868,7 +1896,7 @@ int mgcfd_get_flux_variant(void) { return R.flux_variant; }
 int mgcfd_sync(void)
 {
     if (!R.inited) return fail("mgcfd_sync: not initialised");
-    HIPCHK(hipStreamSynchronize(R.stream));
+    HIPCHK(hipStreamSynchronize(cur_stream()));
     return 0;
 }
 
@@ -1891,7 +1919,7 @@ int op_par_loop_initialize_variables_kernel(const char* name, op_set set, op_arg
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_WRITE"));
-    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), R.k, 1);
+    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), R.k, 1);
     touch(a0.dat);
     return check_launch();
 }
@@ -1902,7 +1930,7 @@ int op_par_loop_zero_5d_array_kernel(const char* name, op_set set, op_arg a0)
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 array[5] OP_WRITE"));
-    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), R.k, 0);
+    if (set->size) k_fill5<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), R.k, 0);
     touch(a0.dat);
     return check_launch();
 }
@@ -1913,7 +1941,7 @@ int op_par_loop_zero_1d_array_kernel(const char* name, op_set set, op_arg a0)
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 1, false, L, "arg0 array[1] OP_WRITE"));
-    if (set->size) k_fill<<<nblk(set->size), kBlock, 0, R.stream>>>((size_t)set->size, dptr(a0.dat), 0.0);
+    if (set->size) k_fill<<<nblk(set->size), kBlock, 0, cur_stream()>>>((size_t)set->size, dptr(a0.dat), 0.0);
     touch(a0.dat);
     return check_launch();
 }
@@ -1936,11 +1964,11 @@ int op_par_loop_calculate_cell_volumes(const char* name, op_set set, op_arg a0, 
     DatImpl* wi = D(a2.dat);
     if (!wi->d_tmp) HIPCHK(hipMalloc(&wi->d_tmp, sizeof(double) * ne));
     TRY(build_inverse(a0.map));
-    k_cell_volumes_edge<<<nblk(ne), kBlock, 0, R.stream>>>(ne, (const int2*)M(a0.map)->d_map, dptr(a0.dat),
+    k_cell_volumes_edge<<<nblk(ne), kBlock, 0, cur_stream()>>>(ne, (const int2*)M(a0.map)->d_map, dptr(a0.dat),
                                                            dptr(a2.dat), wi->d_tmp);
     TRY(check_launch());
     const InvPlan& P = M(a0.map)->inv;
-    k_cell_volumes_node<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr,
+    k_cell_volumes_node<<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(P.n_targets, P.d_targets, P.d_ptr,
                                                                     P.d_inc2, wi->d_tmp, dptr(a3.dat));
     touch(a2.dat);
     touch(a3.dat);
@@ -1953,7 +1981,7 @@ int op_par_loop_dampen_ewt(const char* name, op_set set, op_arg a0)
     (void)name;
     TRY(chk_ready(L));
     TRY(direct(a0, set, 3, false, L, "arg0 weights[3] OP_RW"));
-    if (set->size) k_scale<<<nblk(3 * (size_t)set->size), kBlock, 0, R.stream>>>(3 * (size_t)set->size, dptr(a0.dat), 1e-7);
+    if (set->size) k_scale<<<nblk(3 * (size_t)set->size), kBlock, 0, cur_stream()>>>(3 * (size_t)set->size, dptr(a0.dat), 1e-7);
     touch(a0.dat);
     return check_launch();
 }
@@ -1965,7 +1993,7 @@ int op_par_loop_copy_double_kernel(const char* name, op_set set, op_arg a0, op_a
     TRY(chk_ready(L));
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_READ"));
     TRY(direct(a1, set, 5, false, L, "arg1 old_variables[5] OP_WRITE"));
-    if (set->size) k_copy<<<nblk(5 * (size_t)set->size), kBlock, 0, R.stream>>>(5 * (size_t)set->size, dptr(a0.dat), dptr(a1.dat), nullptr);
+    if (set->size) k_copy<<<nblk(5 * (size_t)set->size), kBlock, 0, cur_stream()>>>(5 * (size_t)set->size, dptr(a0.dat), dptr(a1.dat), nullptr);
     touch(a1.dat);
     return check_launch();
 }
@@ -1979,7 +2007,7 @@ int op_par_loop_calculate_dt_kernel(const char* name, op_set set, op_arg a0, op_
     TRY(direct(a1, set, 1, false, L, "arg1 volumes[1] OP_READ"));
     TRY(direct(a2, set, 1, false, L, "arg2 dt[1] OP_WRITE"));
     if (set->size)
-        k_calc_dt<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
+        k_calc_dt<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat),
                                                             nullptr, nullptr, MinPub{});
     touch(a2.dat);
     return check_launch();
@@ -1998,8 +2026,8 @@ int op_par_loop_get_min_dt_kernel(const char* name, op_set set, op_arg a0, op_ar
     if (n == 0) return 0;
     const int nb = (int)nblk(n);
     TRY(ensure_part(nb));
-    k_min_part<<<nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), R.d_part);
-    k_min_final<<<1, kBlock, 0, R.stream>>>(nb, R.d_part, init, R.d_scal, nullptr);
+    k_min_part<<<nb, kBlock, 0, cur_stream()>>>(n, dptr(a0.dat), R.d_part);
+    k_min_final<<<1, kBlock, 0, cur_stream()>>>(nb, R.d_part, init, R.d_scal, nullptr);
     TRY(check_launch());
     double m;
     TRY(read_scalar(R.d_scal, &m));
@@ -2027,8 +2055,8 @@ int mgcfd_get_min_dt_noalloc(const char* name, op_set set, op_arg a0, op_arg a1)
                     (long long)kNoAllocBlocks * kBlock);
     double* ptr = scratch_dev[R.device];
     if (!ptr) return fail("%s: no scratch address for device %d", L, R.device);
-    k_min_part<<<(unsigned)nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), ptr);
-    k_min_final<<<1, kBlock, 0, R.stream>>>((int)nb, ptr, init, ptr + kNoAllocBlocks, nullptr);
+    k_min_part<<<(unsigned)nb, kBlock, 0, cur_stream()>>>(n, dptr(a0.dat), ptr);
+    k_min_final<<<1, kBlock, 0, cur_stream()>>>((int)nb, ptr, init, ptr + kNoAllocBlocks, nullptr);
     TRY(check_launch());
     double m;
     TRY(read_scalar(ptr + kNoAllocBlocks, &m));
@@ -2049,7 +2077,7 @@ int op_par_loop_compute_step_factor_kernel(const char* name, op_set set, op_arg 
     double m;
     std::memcpy(&m, a2.data, sizeof(double));
     if (set->size)
-        k_step_factor<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a1.dat), nullptr, m, dptr(a3.dat), nullptr);
+        k_step_factor<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a1.dat), nullptr, m, dptr(a3.dat), nullptr);
     touch(a3.dat);
     return check_launch();
 }
@@ -2102,7 +2130,7 @@ int op_par_loop_time_step_kernel(const char* name, op_set set, op_arg a0, op_arg
     int rk;
     std::memcpy(&rk, a0.data, sizeof(int));
     if (set->size)
-        k_time_step<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, rk, dptr(a1.dat), dptr(a2.dat), dptr(a3.dat),
+        k_time_step<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, rk, dptr(a1.dat), dptr(a2.dat), dptr(a3.dat),
                                                               dptr(a4.dat), nullptr);
     touch(a2.dat);
     touch(a4.dat);
@@ -2127,7 +2155,7 @@ int op_par_loop_unstructured_stream_kernel(const char* name, op_set set, op_arg 
     TRY(build_inverse(a0.map));
     const InvPlan& P = M(a0.map)->inv;
     if (P.n_targets)
-        k_stream_gather<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp,
+        k_stream_gather<<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2, wp,
                                                                     dptr(a0.dat), dptr(a3.dat));
     touch(a3.dat);
     return check_launch();
@@ -2142,7 +2170,7 @@ int op_par_loop_residual_kernel(const char* name, op_set set, op_arg a0, op_arg 
     TRY(direct(a1, set, 5, false, L, "arg1 variables[5] OP_READ"));
     TRY(direct(a2, set, 5, false, L, "arg2 residuals[5] OP_WRITE"));
     if (set->size)
-        k_residual<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat), nullptr, nullptr, nullptr);
+        k_residual<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), dptr(a1.dat), dptr(a2.dat), nullptr, nullptr, nullptr);
     touch(a2.dat);
     return check_launch();
 }
@@ -2160,8 +2188,8 @@ int op_par_loop_calc_rms_kernel(const char* name, op_set set, op_arg a0, op_arg 
     if (n == 0) return 0;
     const int nb = (int)nblk(n);
     TRY(ensure_part(nb));
-    k_sumsq_part<<<nb, kBlock, 0, R.stream>>>(n, dptr(a0.dat), R.d_part);
-    k_sum_final<<<1, kBlock, 0, R.stream>>>(nb, R.d_part, init, R.d_scal + 1, n, nullptr, nullptr, nullptr, nullptr);
+    k_sumsq_part<<<nb, kBlock, 0, cur_stream()>>>(n, dptr(a0.dat), R.d_part);
+    k_sum_final<<<1, kBlock, 0, cur_stream()>>>(nb, R.d_part, init, R.d_scal + 1, n, nullptr, nullptr, nullptr, nullptr);
     TRY(check_launch());
     double s;
     TRY(read_scalar(R.d_scal + 1, &s));
@@ -2171,10 +2199,10 @@ int op_par_loop_calc_rms_kernel(const char* name, op_set set, op_arg a0, op_arg 
 
 static int count_loop(int n, op_dat d, int* out, int which)
 {
-    k_fill_i<<<1, kBlock, 0, R.stream>>>(1, R.d_iscal, 0);
+    k_fill_i<<<1, kBlock, 0, cur_stream()>>>(1, R.d_iscal, 0);
     if (n) {
-        if (which == 0) k_count_bad<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), R.d_iscal);
-        else k_count_pos<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(d), R.d_iscal);
+        if (which == 0) k_count_bad<<<nblk(n), kBlock, 0, cur_stream()>>>(n, dptr(d), R.d_iscal);
+        else k_count_pos<<<nblk(n), kBlock, 0, cur_stream()>>>(n, dptr(d), R.d_iscal);
         TRY(check_launch());
     }
     return read_iscalar(R.d_iscal, out);
@@ -2206,7 +2234,7 @@ int op_par_loop_up_pre_kernel(const char* name, op_set set, op_arg a0, op_arg a1
               D(a1.dat)->is_int && a1.dat->set == a0.map->to;
     TRY(need(ok, L, "arg1 up_scratch int via mg map OP_WRITE"));
     if (set->size)
-        k_up_pre<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, M(a0.map)->d_map, dptr(a0.dat),
+        k_up_pre<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, M(a0.map)->d_map, dptr(a0.dat),
                                                            iptr(a1.dat), nullptr);
     touch(a0.dat);
     touch(a1.dat);
@@ -2226,7 +2254,7 @@ int op_par_loop_up_kernel(const char* name, op_set set, op_arg a0, op_arg a1, op
     TRY(build_inverse(a1.map));
     const InvPlan& P = M(a1.map)->inv;
     if (P.n_targets)
-        k_up_gather<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
+        k_up_gather<<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc1,
                                                                 dptr(a0.dat), dptr(a1.dat), iptr(a2.dat), 0, nullptr);
     touch(a1.dat);
     touch(a2.dat);
@@ -2241,7 +2269,7 @@ int op_par_loop_up_post_kernel(const char* name, op_set set, op_arg a0, op_arg a
     TRY(direct(a0, set, 5, false, L, "arg0 variables[5] OP_RW"));
     TRY(direct(a1, set, 1, true, L, "arg1 up_scratch int OP_READ"));
     if (set->size)
-        k_up_post<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), iptr(a1.dat), nullptr);
+        k_up_post<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), iptr(a1.dat), nullptr);
     touch(a0.dat);
     return check_launch();
 }
@@ -2258,7 +2286,7 @@ int op_par_loop_down_kernel(const char* name, op_set set, op_arg a0, op_arg a1, 
     TRY(indirect(a4, set, 0, 1, 3, L, "arg4 coarse coords via mg map"));
     TRY(need(a3.map == a4.map, L, "same mg map"));
     if (set->size)
-        k_down<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, M(a3.map)->d_map, dptr(a0.dat), dptr(a1.dat),
+        k_down<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, M(a3.map)->d_map, dptr(a0.dat), dptr(a1.dat),
                                                          dptr(a2.dat), dptr(a3.dat), dptr(a4.dat), nullptr);
     touch(a0.dat);
     return check_launch();
@@ -2273,8 +2301,8 @@ int op_par_loop_down_v2_kernel_pre(const char* name, op_set set, op_arg a0, op_a
     TRY(direct(a1, set, 5, false, L, "arg1 prolonged residuals[5] OP_WRITE"));
     const size_t n = (size_t)set->size;
     if (n) {
-        k_fill<<<nblk(n), kBlock, 0, R.stream>>>(n, dptr(a0.dat), 0.0);
-        k_fill<<<nblk(5 * n), kBlock, 0, R.stream>>>(5 * n, dptr(a1.dat), 0.0);
+        k_fill<<<nblk(n), kBlock, 0, cur_stream()>>>(n, dptr(a0.dat), 0.0);
+        k_fill<<<nblk(5 * n), kBlock, 0, cur_stream()>>>(5 * n, dptr(a1.dat), 0.0);
     }
     touch(a0.dat);
     touch(a1.dat);
@@ -2309,7 +2337,7 @@ int op_par_loop_down_v2_kernel(const char* name, op_set set, op_arg a0, op_arg a
     TRY(build_inverse(en));
     const InvPlan& P = M(en)->inv;
     if (P.n_targets)
-        k_down_v2<<<nblk(P.n_targets), kBlock, 0, R.stream>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2,
+        k_down_v2<<<nblk(P.n_targets), kBlock, 0, cur_stream()>>>(P.n_targets, P.d_targets, P.d_ptr, P.d_inc2,
                                                                M(em)->d_map, dptr(a0.dat), dptr(a2.dat),
                                                                dptr(a4.dat), dptr(a6.dat), dptr(a8.dat));
     touch(a6.dat);
@@ -2327,7 +2355,7 @@ int op_par_loop_down_v2_kernel_post(const char* name, op_set set, op_arg a0, op_
     TRY(direct(a2, set, 5, false, L, "arg2 residuals[5] OP_READ"));
     TRY(direct(a3, set, 5, false, L, "arg3 variables[5] OP_INC"));
     if (set->size)
-        k_down_v2_post<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat),
+        k_down_v2_post<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), dptr(a1.dat),
                                                                  dptr(a2.dat), dptr(a3.dat));
     touch(a3.dat);
     return check_launch();
@@ -2342,7 +2370,7 @@ int op_par_loop_identify_differences(const char* name, op_set set, op_arg a0, op
     TRY(direct(a1, set, 5, false, L, "arg1 master values[5] OP_READ"));
     TRY(direct(a2, set, 5, false, L, "arg2 differences[5] OP_WRITE"));
     if (set->size)
-        k_identify_differences<<<nblk(set->size), kBlock, 0, R.stream>>>(set->size, dptr(a0.dat), dptr(a1.dat),
+        k_identify_differences<<<nblk(set->size), kBlock, 0, cur_stream()>>>(set->size, dptr(a0.dat), dptr(a1.dat),
                                                                          dptr(a2.dat), nullptr);
     touch(a2.dat);
     return check_launch();
