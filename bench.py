@@ -527,6 +527,10 @@ def main():
     # reference's run scripts ask (run-templates/local.sh:3-4); set before the
     # reference-kernel library (and its OpenMP runtime) loads
     os.environ.setdefault("OMP_PROC_BIND", "true")
+    # OpenMP binds this (its initial) thread to one core; every thread the
+    # library starts later (plan building, served admissions) inherits the
+    # mask, so it is put back after each CPU leg
+    affinity = os.sched_getaffinity(0)
     host = host_cpu()
     log(f"bench: host CPU {host}")
     log("bench: generating the 300K-node / 930K-edge mesh")
@@ -601,12 +605,14 @@ def main():
         var = rt.fetch(solvers[0].lv[0].variables)
         w = rt.fetch(solvers[0].lv[0].edge_weights)
         cpu = cpu_baseline(mesh0, (var, w), args.cpu_seconds, host)
+        os.sched_setaffinity(0, affinity)
     shard = None if args.no_shard else bench_shard(rt, best, args.steps, args.warmup)
     solve = None
     if not args.no_solve:
         solve, solve_mesh = bench_solve(rt)
         if cpu is not None:
             cpu["solve"] = cpu_solve(solve_mesh, host)
+            os.sched_setaffinity(0, affinity)
             log(f"bench: cpu solve: {cpu['solve']}")
         del solve_mesh
     # the back-to-back FP64 baseline runs the fastest FP64 flux within 1e-12
