@@ -25,7 +25,6 @@
 #include <deque>
 #include <mutex>
 #include <numeric>
-#include <shared_mutex>
 #include <string>
 #include <vector>
 
