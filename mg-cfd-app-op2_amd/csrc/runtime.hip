@@ -123,7 +123,7 @@ struct InvPlan {
                                      // per CU (autotuned)
     int pt_rs_set = 0;               // FP64 record values of the edge-flux kernel: 0 by the rule below, or 5 / 6
     bool bk_built = false, pt_built = false;
-    std::vector<int> h_ptr;          // host CSR kept for re-planning (dim-2 dense maps)
+    std::vector<int> h_ptr;          // host CSR over every target-set element, kept for re-planning (dim-2 maps)
     std::vector<int2> h_inc2;
 };
 
@@ -1059,10 +1059,16 @@ static int build_inverse(op_map m)
     if (dim == 2) {
         HIPCHK(hipMalloc(&P.d_inc2, sizeof(int2) * std::max<size_t>(1, inc2.size())));
         if (!inc2.empty()) HIPCHK(hipMemcpy(P.d_inc2, inc2.data(), sizeof(int2) * inc2.size(), hipMemcpyHostToDevice));
-        if (P.dense) {          // flux plans are built on first use (ensure_flux_plan)
+        // flux plans are built on first use (ensure_flux_plan) from a CSR
+        // over every node of the target set: a node no edge touches has an
+        // empty row (the incidences are the same, in the same order)
+        if (P.dense) {
             P.h_ptr = std::move(ptr);
-            P.h_inc2 = std::move(inc2);
+        } else {
+            P.h_ptr.assign(nt + 1, 0);
+            for (int t = 0; t < nt; t++) P.h_ptr[t + 1] = P.h_ptr[t] + cnt[t];
         }
+        P.h_inc2 = std::move(inc2);
     } else {
         HIPCHK(hipMalloc(&P.d_inc1, sizeof(int) * std::max<size_t>(1, inc1.size())));
         if (!inc1.empty()) HIPCHK(hipMemcpy(P.d_inc1, inc1.data(), sizeof(int) * inc1.size(), hipMemcpyHostToDevice));
@@ -1099,7 +1105,7 @@ static int set_part_nodes(op_map m, int nodes)
     if (nodes < 16 || nodes > kPart) return fail("set_part_nodes: %d nodes per part outside [16, %d]", nodes, kPart);
     TRY(build_inverse(m));
     InvPlan& P = M(m)->inv;
-    if (!P.dense || P.h_ptr.empty()) return fail("set_part_nodes: map %s has no part plan", m->name);
+    if (P.h_ptr.empty()) return fail("set_part_nodes: map %s has no part plan", m->name);
     if (P.pt_psize == nodes && P.pt_built) return 0;
     HIPCHK(hipStreamSynchronize(cur_stream()));
     free_partplan(P);
@@ -1160,7 +1166,7 @@ static int ensure_flux_plan(op_map m, int variant)
 {
     TRY(build_inverse(m));
     InvPlan& P = M(m)->inv;
-    if (!P.dense || P.h_ptr.empty()) return 0;
+    if (P.h_ptr.empty()) return 0;
     const bool gather = variant == MGCFD_FLUX_GATHER_EXACT || variant == MGCFD_FLUX_GATHER_FAST;
     const bool part = variant == MGCFD_FLUX_PART_COLOURED || variant == MGCFD_FLUX_PART_ATOMIC ||
                       variant == MGCFD_FLUX_MIXED_COLOURED || variant == MGCFD_FLUX_MIXED_ATOMIC;
@@ -1456,7 +1462,7 @@ static int launch_flux(int variant, op_map map, op_dat var, op_dat w, op_dat flu
     if (P.n_targets == 0) return 0;
     TRY(refresh_plan_weights(variant, P, w));
     if (part_variant(variant) || mixed_variant(variant)) {
-        if (P.pt_parts == 0) return fail("edge-flux variant %d needs an edge map onto every node of its set", variant);
+        if (P.pt_parts == 0) return fail("edge-flux variant %d needs a two-column edge map", variant);
         if (mixed_variant(variant))
             return launch_part<float>(variant == MGCFD_FLUX_MIXED_COLOURED, P, P.d_pt_w32, P.pt_lds32, var, flux,
                                       status, fresh);
