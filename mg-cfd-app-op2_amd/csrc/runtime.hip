@@ -92,6 +92,8 @@ struct InvPlan {
     unsigned* d_bk_jslot = nullptr;   // per entry: LDS slot of the other endpoint | b-role << 31
     int* d_bk_eid = nullptr;     // per entry: internal edge id (-1 = padding)
     double* d_bk_w3 = nullptr;   // per entry: weight components w0 | w1 | w2 (SoA)
+    int* d_bk_nptr = nullptr;    // CSR pointer over every node of the target set (a node's degree; d_ptr
+                                 // counts targets only when some node has no incidence)
     const void* bk_wdat = nullptr;    // weight dat the packed weights came from
     unsigned bk_wver = 0;
     int bk_max_halo = 0;
@@ -248,6 +250,20 @@ static int check_launch()
 // a part finds most neighbours of its nodes inside the part.
 // Morton (Z-order) key of each node of [lo, hi): coordinates quantised to
 // 21 bits per axis over the range's bounding box, bits interleaved.
+// Bit b of a 21-bit value moved to bit 3b.
+static inline uint64_t spread3(uint64_t x)
+{
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+// Z-order of the nodes [lo, hi): 21 bits per axis over the bounding box,
+// axis 0 most significant within each bit triple; ties keep node order.
 static void morton_order(const std::vector<double>& geo, int lo, int hi, std::vector<int>& order)
 {
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
@@ -264,10 +280,7 @@ static void morton_order(const std::vector<double>& geo, int lo, int hi, std::ve
             const double f = span > 0 ? (geo[3 * (size_t)i + a] - mn[a]) / span : 0.0;
             q[a] = (uint64_t)std::min(2097151.0, std::max(0.0, f * 2097151.0));
         }
-        uint64_t k = 0;
-        for (int b = 20; b >= 0; b--)
-            for (int a = 0; a < 3; a++) k = (k << 1) | ((q[a] >> b) & 1u);
-        key[i - lo] = {k, i};
+        key[i - lo] = {(spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]), i};
     }
     std::sort(key.begin(), key.end());
     for (int i = lo; i < hi; i++) order[i] = key[i - lo].second;
@@ -286,14 +299,18 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
         }
     }
     for (int i = 0; i < n; i++) ptr[i + 1] += ptr[i];
-    std::vector<int> adj(ptr[n]);
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (size_t m = 0; m < emaps.size(); m++) {
-        const auto& h = emaps[m]->h_orig;
-        for (int e = 0; e < from_sizes[m]; e++) {
-            int a = h[2 * (size_t)e], b = h[2 * (size_t)e + 1];
-            adj[fill[a]++] = b;
-            adj[fill[b]++] = a;
+    // the adjacency lists serve the graph bisection only; the geometric
+    // order needs the degrees alone
+    std::vector<int> adj(geo.empty() ? ptr[n] : 0);
+    if (geo.empty()) {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (size_t m = 0; m < emaps.size(); m++) {
+            const auto& h = emaps[m]->h_orig;
+            for (int e = 0; e < from_sizes[m]; e++) {
+                int a = h[2 * (size_t)e], b = h[2 * (size_t)e + 1];
+                adj[fill[a]++] = b;
+                adj[fill[b]++] = a;
+            }
         }
     }
     std::vector<int> order(n);
@@ -703,6 +720,9 @@ static void blocked_part(int lo0, int hi0, const std::vector<int>& ptr, const st
 static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
 {
     HostTimer tm("blocked plan", m->name);
+    // the kernel reads a node's degree from ptr at the node's index
+    if ((long long)ptr.size() != (long long)m->to->size + 1)
+        return fail("internal error: blocked plan of %s needs a CSR over every node", m->name);
     const std::vector<int> pb = set_parts(m->to);
     const int np = (int)pb.size() - 1;
     std::vector<BlkOut> outs(std::max(np, 0));
@@ -754,6 +774,7 @@ static int build_blocked(InvPlan& P, op_map m, const std::vector<int>& ptr, cons
     TRY(up(P.d_bk_jpref, jpref));
     TRY(up(P.d_bk_jslot, jslot));
     TRY(up(P.d_bk_eid, eid));
+    TRY(up(P.d_bk_nptr, ptr));
     HIPCHK(hipMalloc(&P.d_bk_w3, sizeof(double) * 3 * std::max<size_t>(1, eid.size())));
     P.bk_blocks = (int)blocks.size();
     P.bk_entries = (int)eid.size();
@@ -924,6 +945,8 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
 static int build_partplan(InvPlan& P, op_map m, const std::vector<int>& ptr, const std::vector<int2>& inc2)
 {
     HostTimer tm("part plan", m->name);
+    if ((long long)ptr.size() != (long long)m->to->size + 1)
+        return fail("internal error: part plan of %s needs a CSR over every node", m->name);
     const int nt = m->to->size;
     const std::vector<int> pb = set_parts(m->to);
     const int np = (int)pb.size() - 1;
@@ -1121,7 +1144,8 @@ static void free_invplan(InvPlan& P)
     if (P.d_nptr && P.d_nptr != P.d_ptr) hipFree(P.d_nptr);
     for (void* p : {(void*)P.d_targets, (void*)P.d_ptr, (void*)P.d_inc2, (void*)P.d_inc1, (void*)P.d_bk_bnode,
                     (void*)P.d_bk_hptr, (void*)P.d_bk_halo, (void*)P.d_bk_jbase, (void*)P.d_bk_pbase,
-                    (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w3})
+                    (void*)P.d_bk_jpref, (void*)P.d_bk_jslot, (void*)P.d_bk_eid, (void*)P.d_bk_w3,
+                    (void*)P.d_bk_nptr})
         if (p) hipFree(p);
     free_partplan(P);
     P = InvPlan();
@@ -1294,11 +1318,11 @@ static int launch_blocked_pf(bool exact, const InvPlan& P, op_dat var, op_dat fl
 {
     if (exact)
         k_flux_blocked_pf<true, 4><<<P.bk_blocks, kBlock, P.bk_lds, cur_stream()>>>(
-            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
+            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_bk_nptr, P.d_bk_jslot,
             P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     else
         k_flux_blocked_pf<false, 4><<<P.bk_blocks, kBlock, P.bk_lds, cur_stream()>>>(
-            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_ptr, P.d_bk_jslot,
+            P.d_bk_bnode, P.d_bk_hptr, P.d_bk_halo, P.d_bk_jbase, P.d_bk_pbase, P.d_bk_jpref, P.d_bk_nptr, P.d_bk_jslot,
             P.d_bk_w3, P.bk_entries, dptr(var), dptr(flux), R.k, status, fresh);
     return check_launch();
 }
@@ -1668,9 +1692,12 @@ static op_dat decl_dat(op_set set, int dim, const char* type, int size, const ch
         // 'int' dats declared with 8-byte elements (euler3d.cpp:405) are read
         // through their first 4 bytes, as the kernels see them.
         const size_t in_esz = is_int ? sizeof(int) : sizeof(double);
-        di->pending.resize((size_t)set->size * dim * in_esz);
-        for (size_t q = 0; q < (size_t)set->size * dim; q++)
-            std::memcpy(di->pending.data() + q * in_esz, data + q * (size_t)size, in_esz);
+        const size_t nel = (size_t)set->size * dim;
+        di->pending.resize(nel * in_esz);
+        if ((size_t)size == in_esz)
+            std::memcpy(di->pending.data(), data, nel * in_esz);
+        else
+            for (size_t q = 0; q < nel; q++) std::memcpy(di->pending.data() + q * in_esz, data + q * (size_t)size, in_esz);
     }
     RtLock lk(g_rt_mu);
     R.dats.push_back(op_dat_core{});
