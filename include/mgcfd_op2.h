@@ -381,19 +381,30 @@ int mgcfd_batch_destroy(mgcfd_batch b);
  * target (stop at the first cycle with rms[c] <= rms_drop * rms[0]; 0 = run
  * to the cap).  Requests are admitted in arrival order while the estimated
  * device memory of the admitted ones stays within hbm_budget_bytes and at
- * most max_active run at once; each runs as its own device-resident context
- * on its own stream, its cycles issued in chunks of cycles_per_chunk graph
- * replays, so requests that arrive while others run overlap with them on the
- * device.  A request's result is the same alone and in any mix.
+ * most max_active contexts run at once.  The request at the head of the
+ * queue is admitted together with the compatible requests queued behind it
+ * (same flux variant, level count, cycle cap and RMS-drop target; see
+ * mgcfd_server_set_grouping): the group runs as one device-resident context
+ * whose mesh segments are its members, on its own stream, so one launch per
+ * loop serves them all; its cycles are issued as plain launches in chunks of
+ * cycles_per_chunk.  Groups that arrive while others run overlap with them
+ * on the device.  A request's result is the same alone and in any mix: a
+ * member that converges has its state copied at that cycle, and one that
+ * aborts (negative dt, NaN) is served again alone.
  * The caller drives the loop with mgcfd_server_progress (non-blocking).
  */
 typedef struct mgcfd_server_s* mgcfd_server;
 int mgcfd_server_create(double hbm_budget_bytes, int max_active, int cycles_per_chunk, mgcfd_server* out);
+/* Group limits of the server: at most max_group requests (default 8) and
+ * max_group_nodes level-0 nodes (default 4,000,000; a lone request above it
+ * still runs) per group.  max_group = 1 runs every request in a context of
+ * its own. */
+int mgcfd_server_set_grouping(mgcfd_server s, int max_group, long long max_group_nodes);
 int mgcfd_server_submit(mgcfd_server s, const mgcfd_mesh* mesh, double mach, double aoa_deg, int flux_variant,
                         int max_cycles, double rms_drop, int* request_id);
 /* Finishes, continues and admits requests; *n_active / *n_queued (nullable)
  * receive the counts afterwards. */
-int mgcfd_server_progress(mgcfd_server s, int* n_active, int* n_queued);
+int mgcfd_server_progress(mgcfd_server s, int* n_active, int* n_queued);   /* n_active: requests */
 /* *state: 0 queued, 1 running, 2 done (then status -- 0 ran to the cap,
  * 1 negative min dt, 2 NaN/Inf, 3 converged --, cycles run, latency from
  * submit to completion in seconds, level-0 variables [N][5] and the RMS

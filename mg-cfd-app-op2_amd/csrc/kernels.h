@@ -1986,14 +1986,42 @@ __global__ void k_bump(int* c, const int* __restrict__ status)
 // Convergence stop of served solves, run after the cycle counter moved on:
 // segment s whose L0 RMS of the cycle that just ran has dropped to 'drop'
 // times its first cycle's gets status 3, which halts every later kernel of a
-// one-segment context (the state stays that of the converged cycle).
+// one-segment context (the state stays that of the converged cycle).  In a
+// context of several segments nothing halts: the segment's cycle count goes
+// to segcyc[s], k_snap_converged copies its state this cycle, and the next
+// call marks it copied (snapflag).
 __global__ void k_converge(int nseg, int* __restrict__ sstatus, const double* __restrict__ rms, int cap,
-                           const int* __restrict__ cycle, double drop)
+                           const int* __restrict__ cycle, double drop, int* __restrict__ snapflag,
+                           int* __restrict__ segcyc)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg || sstatus[s] != 0) return;
+    if (s >= nseg) return;
+    if (sstatus[s] == 3) {
+        if (snapflag) snapflag[s] = 1;
+        return;
+    }
+    if (sstatus[s] != 0) return;
     const int c = *cycle - 1;       // the cycle that just ran
-    if (c >= 1 && c < cap && rms[(size_t)s * cap + c] <= drop * rms[(size_t)s * cap]) sstatus[s] = 3;
+    if (c >= 1 && c < cap && rms[(size_t)s * cap + c] <= drop * rms[(size_t)s * cap]) {
+        sstatus[s] = 3;
+        if (segcyc) segcyc[s] = c + 1;
+    }
+}
+
+// Level-0 state of the segments that converged in the cycle that just ran
+// (status 3, not yet copied), one workgroup per segment block {segment, lo,
+// hi}: the result of a served request that keeps running with the other
+// members of its group.
+__global__ __launch_bounds__(kBlock) void k_snap_converged(int nblocks, const int4* __restrict__ blk,
+                                                           const int* __restrict__ sstatus,
+                                                           const int* __restrict__ snapflag,
+                                                           const double* __restrict__ var, double* __restrict__ snap)
+{
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const int4 q = blk[b];
+    if (sstatus[q.x] != 3 || snapflag[q.x]) return;
+    for (int i = 5 * q.y + threadIdx.x; i < 5 * q.z; i += kBlock) snap[i] = var[i];
 }
 
 }  // namespace mgcfd

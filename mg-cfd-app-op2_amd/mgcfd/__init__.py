@@ -425,12 +425,15 @@ class Batch:
 class Server:
     """Continuous-batching request loop (include/mgcfd_op2.h, mgcfd_server_*)."""
 
-    def __init__(self, rt: "Runtime", budget_bytes: float = 200e9, max_active: int = 8, chunk: int = 10):
+    def __init__(self, rt: "Runtime", budget_bytes: float = 200e9, max_active: int = 8, chunk: int = 10,
+                 max_group: int = 8, max_group_nodes: int = 4_000_000):
         self.rt, self.L = rt, rt.L
         self.h = C.c_void_p()
         self.n0 = {}
         _check(self.L.mgcfd_server_create(C.c_double(budget_bytes), max_active, chunk, C.byref(self.h)),
                "mgcfd_server_create")
+        _check(self.L.mgcfd_server_set_grouping(self.h, max_group, C.c_longlong(max_group_nodes)),
+               "mgcfd_server_set_grouping")
 
     def submit(self, mesh: list[dict], mach: float, aoa_deg: float, variant: str = "gather_exact",
                max_cycles: int = 200, rms_drop: float = 0.0) -> int:

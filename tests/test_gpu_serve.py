@@ -137,3 +137,88 @@ def test_convergence_stop_is_exact(rt, meshes):
     assert all(r["rms"][k] > drop * r["rms"][0] for k in range(1, c - 1))
     ref = Oracle("orc").solve(meshes[0], c, 0.5, 3.0)
     assert rel(r["variables"], ref["levels"][0]["variables"]) <= 1e-12
+
+
+def serve_alone(rt, mesh, mach, aoa, var, **kw):
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=1, chunk=3, max_group=1)
+    j = srv.submit(mesh, mach, aoa, var, **kw)
+    r = drain(srv, [j])[j]
+    srv.close()
+    return r
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def test_queued_requests_share_a_context(rt, meshes):
+    """Requests queued behind a busy server are admitted as groups: the
+    compatible ones (same variant, levels, cycle cap and drop) share one
+    context, whose launches serve them all (identical device intervals), and
+    each result is bitwise the request served alone."""
+    reqs = [(0, 0.5, 3.0, "part_coloured"), (2, 0.4, 1.0, "mixed_coloured"), (3, 0.6, -1.0, "part_coloured"),
+            (1, 0.7, 2.0, "part_coloured"), (0, 0.3, 4.0, "mixed_coloured")]
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=1, chunk=5)
+    ids = [srv.submit(meshes[mi], mach, aoa, var, max_cycles=CYCLES) for mi, mach, aoa, var in reqs]
+    res = drain(srv, ids, max_active=3)
+    spans = [srv.timing(i) for i in ids]
+    srv.close()
+    print("device intervals (ms):", spans)
+    assert spans[0] == spans[2] == spans[3]          # the three FP64 requests: one group
+    assert spans[1] == spans[4] and spans[1] != spans[0]
+    for k, (mi, mach, aoa, var) in enumerate(reqs):
+        r = res[ids[k]]
+        a = serve_alone(rt, meshes[mi], mach, aoa, var, max_cycles=CYCLES)
+        assert r["status"] == a["status"] == 0 and r["cycles"] == a["cycles"] == CYCLES
+        assert same(r["variables"], a["variables"]), k
+        assert same(r["rms"], a["rms"]), k
+
+
+def test_grouped_members_stop_at_their_own_convergence(rt, meshes):
+    """Members of one group converge at different cycles; each one's state is
+    copied on the device at its own convergence cycle while the others run
+    on, so status, cycle count, RMS history and state equal the request
+    served alone (whose convergence halts its context)."""
+    drop = 1.0 - 1.5e-6
+    reqs = [(0, 0.5, 3.0), (2, 0.5, 3.0), (3, 0.45, 2.0), (1, 0.5, 3.0)]
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=1, chunk=4)
+    ids = [srv.submit(meshes[mi], mach, aoa, "gather_exact", max_cycles=40, rms_drop=drop) for mi, mach, aoa in reqs]
+    res = drain(srv, ids)
+    spans = [srv.timing(i) for i in ids]
+    srv.close()
+    cycles = [res[i]["cycles"] for i in ids]
+    print("cycles per member:", cycles, "statuses:", [res[i]["status"] for i in ids])
+    assert spans[0] == spans[1] == spans[2] == spans[3]     # all queued together: one group
+    # the oracle's RMS histories put the stops at 6, 37 and 14 cycles, and
+    # the third member at its 40-cycle cap
+    assert len(set(cycles)) == 4
+    for k, (mi, mach, aoa) in enumerate(reqs):
+        r = res[ids[k]]
+        a = serve_alone(rt, meshes[mi], mach, aoa, "gather_exact", max_cycles=40, rms_drop=drop)
+        assert (r["status"], r["cycles"]) == (a["status"], a["cycles"]), k
+        assert same(r["variables"], a["variables"]), k
+        assert same(r["rms"], a["rms"]), k
+
+
+def test_member_abort_is_served_alone(rt, meshes):
+    """A member whose solve aborts (a NaN edge weight: bad values, status 2)
+    is served again alone, so its result is the lone request's; the other
+    members of its group are unaffected."""
+    bad = [dict(l) for l in meshes[2]]
+    w = np.array(bad[0]["edge_weights"], dtype=np.float64, copy=True)
+    w[17, 1] = np.nan
+    bad[0]["edge_weights"] = w
+    reqs = [(meshes[0], 0.5, 3.0), (meshes[3], 0.5, 3.0), (bad, 0.5, 3.0), (meshes[1], 0.6, 1.0)]
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=1, chunk=2)
+    ids = [srv.submit(m, mach, aoa, "part_coloured", max_cycles=CYCLES) for m, mach, aoa in reqs]
+    res = drain(srv, ids)
+    spans = [srv.timing(i) for i in ids]
+    srv.close()
+    print("statuses:", [res[i]["status"] for i in ids], "intervals:", spans)
+    assert res[ids[2]]["status"] == 2
+    assert spans[0] == spans[1] == spans[3] and spans[2] != spans[1]    # re-served after its group
+    for k, (m, mach, aoa) in enumerate(reqs):
+        r = res[ids[k]]
+        a = serve_alone(rt, m, mach, aoa, "part_coloured", max_cycles=CYCLES)
+        assert (r["status"], r["cycles"]) == (a["status"], a["cycles"]), k
+        assert same(r["variables"], a["variables"]), k
