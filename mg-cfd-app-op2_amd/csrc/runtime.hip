@@ -250,6 +250,23 @@ static int check_launch()
 // a part finds most neighbours of its nodes inside the part.
 // Morton (Z-order) key of each node of [lo, hi): coordinates quantised to
 // 21 bits per axis over the range's bounding box, bits interleaved.
+// n independent host-only tasks (no HIP calls), at most 16 threads.
+template <typename F>
+static void host_tasks(int n, F&& f)
+{
+    if (n <= 1) {
+        if (n == 1) f(0);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(n, 16); t++)
+        pool.emplace_back([&] {
+            for (int i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto& th : pool) th.join();
+}
+
 // Bit b of a 21-bit value moved to bit 3b.
 static inline uint64_t spread3(uint64_t x)
 {
@@ -339,7 +356,7 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
     if (!geo.empty()) {
         // coordinates known (op_partition with a coordinate dat, batched and
         // served meshes): Z-order per segment, cut into kPart-node parts
-        for (size_t k = 0; k + 1 < so.size(); k++) morton_order(geo, so[k], so[k + 1], order);
+        host_tasks((int)so.size() - 1, [&](int k) { morton_order(geo, so[k], so[k + 1], order); });
     } else {
         for (size_t k = so.size() - 1; k-- > 0;) {
             for (int i = so[k]; i < so[k + 1]; i++) seg[i] = next_seg;
@@ -377,13 +394,15 @@ static std::vector<int> partition_order(int n, const std::vector<const MapImpl*>
     // diagonals over a part's threads)
     parts.assign(1, 0);
     for (size_t k = 0; k + 1 < so.size(); k++)
+        for (int lo = so[k]; lo < so[k + 1]; lo += kPart) parts.push_back(std::min(so[k + 1], lo + kPart));
+    host_tasks((int)so.size() - 1, [&](int k) {
         for (int lo = so[k]; lo < so[k + 1]; lo += kPart) {
             const int hi = std::min(so[k + 1], lo + kPart);
             std::stable_sort(order.begin() + lo, order.begin() + hi, [&](int x, int y) {
                 return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y];
             });
-            parts.push_back(hi);
         }
+    });
     return order;   // order[internal] = original
 }
 
@@ -572,19 +591,50 @@ static int freeze()
             lo[e] = a;
             hi[e] = b;
         }
-        auto counting = [&](const std::vector<int>& key1, const std::vector<int>& in, std::vector<int>& out) {
-            std::vector<int> cnt(nt + 1, 0);
-            for (int e : in) cnt[key1[e] + 1]++;
-            for (int i = 0; i < nt; i++) cnt[i + 1] += cnt[i];
-            for (int e : in) out[cnt[key1[e]]++] = e;
+        // counting sort of the elements [e0, e1), whose keys lie in [k0, k1),
+        // into out[e0, e1)
+        auto counting = [&](const std::vector<int>& key1, const int* in, int* out, int e0, int e1, int k0, int k1) {
+            std::vector<int> cnt(k1 - k0 + 1, 0);
+            for (int q = e0; q < e1; q++) cnt[key1[in[q]] - k0 + 1]++;
+            for (int i = 0; i < k1 - k0; i++) cnt[i + 1] += cnt[i];
+            for (int q = e0; q < e1; q++) out[e0 + cnt[key1[in[q]] - k0]++] = in[q];
         };
         std::vector<int> id(s.size), tmp(s.size);
         std::iota(id.begin(), id.end(), 0);
-        counting(hi, id, tmp);
         si->perm.resize(s.size);
-        counting(lo, tmp, si->perm);
-        si->iperm.assign(s.size, 0);
+        // Elements of a segmented target set (batched meshes) that come
+        // segment by segment reference only their segment's nodes, so the
+        // order is the segments' own orders one after another: sorted per
+        // segment, in parallel.  Otherwise one sort over the whole set.
+        const std::vector<int>& tseg = ts->seg;
+        std::vector<int> eb;
+        if (tseg.size() > 2 && s.size > 0) {
+            eb.push_back(0);
+            int k = 0;
+            bool ok = true;
+            for (int e = 0; e < s.size && ok; e++) {
+                while (k + 1 < (int)tseg.size() - 1 && lo[e] >= tseg[k + 1]) {
+                    k++;
+                    eb.push_back(e);
+                }
+                ok = lo[e] >= tseg[k] && hi[e] < tseg[k + 1];
+            }
+            while ((int)eb.size() < (int)tseg.size()) eb.push_back(s.size);
+            if (!ok) eb.clear();
+        }
+        if (!eb.empty()) {
+            host_tasks((int)eb.size() - 1, [&](int k) {
+                counting(hi, id.data(), tmp.data(), eb[k], eb[k + 1], tseg[k], tseg[k + 1]);
+                counting(lo, tmp.data(), si->perm.data(), eb[k], eb[k + 1], tseg[k], tseg[k + 1]);
+            });
+        } else {
+            counting(hi, id.data(), tmp.data(), 0, s.size, 0, nt);
+            counting(lo, tmp.data(), si->perm.data(), 0, s.size, 0, nt);
+        }
+        si->iperm.assign(s.size, -1);
         for (int i = 0; i < s.size; i++) si->iperm[si->perm[i]] = i;
+        for (int i = 0; i < s.size; i++)
+            if (si->iperm[i] < 0) return fail("internal error: element order of set %s is not a permutation", s.name);
         return 0;
     }))
         return rc;
@@ -1643,6 +1693,27 @@ op_map op_decl_map(op_set from, op_set to, int dim, const int* imap, const char*
                      e, mi->h_orig[2 * (size_t)e]);
                 return nullptr;
             }
+    return add_map(from, to, dim, mi, name);
+}
+
+// A map from zero-based entries that the caller has range-checked (the
+// library's own batched meshes): the vector is taken over, with no copy and
+// no OP_MAPS_BASE_INDEX shift (that applies to the user's maps only).
+static op_map decl_map_owned(op_set from, op_set to, int dim, std::vector<int>&& v, const char* name)
+{
+    if (!from || !to || dim < 1 || v.size() != (size_t)from->size * dim) {
+        fail("decl_map %s: bad arguments", name ? name : "");
+        return nullptr;
+    }
+    if (dim == 2)
+        for (int e = 0; e < from->size; e++)
+            if (v[2 * (size_t)e] == v[2 * (size_t)e + 1]) {
+                fail("op_decl_map %s: element %d maps both columns to node %d (self-loop)", name ? name : "", e,
+                     v[2 * (size_t)e]);
+                return nullptr;
+            }
+    MapImpl* mi = new MapImpl();
+    mi->h_orig = std::move(v);
     return add_map(from, to, dim, mi, name);
 }
 
