@@ -58,6 +58,9 @@ FLOP_PER_EDGE = 201             # flux.h:48-207 as written: 186 add/mul + 8 div 
 N_NODES, N_EDGES = 300_000, 930_000
 MACH, AOA = 0.5, 3.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "flux_traffic.json")
+# served requests queued behind a busy server share a context in groups of at
+# most this many requests and level-0 nodes
+SERVE_GROUP, SERVE_GROUP_NODES = 8, 4_000_000
 COPY_BYTES = 512 << 20          # calibration copy: 512 MiB read + 512 MiB written per launch
 # demangled kernel name pattern of each edge-flux variant
 VARIANT_KERNELS = {
@@ -437,8 +440,8 @@ def bench_serving(rt, n_req, rates, seed=50):
     0 = all at once); N lognormal clipped to [10K, 2M] nodes (4 levels);
     Mach/angle as batched; FP64 (PART_COLOURED) / mixed precision
     (MIXED_COLOURED) 50/50; each runs to an L0 RMS of 1e-6 x its first
-    cycle's or 200 cycles; 200 GB admission budget, at most 8 requests on the
-    device at once.  Latency is submit -> result on the host.  The rates run
+    cycle's or 200 cycles; 200 GB admission budget, at most 8 contexts on the
+    device at once, queued compatible requests grouped into shared contexts.  Latency is submit -> result on the host.  The rates run
     from below to past the server's capacity: the last one measures the
     saturated throughput."""
     rng = np.random.default_rng(seed)
@@ -458,7 +461,8 @@ def bench_serving(rt, n_req, rates, seed=50):
         arrivals = np.cumsum(gaps / rate) if rate > 0 else np.zeros(n_req)
         # every request's 200 cycles are issued at admission (one stream of
         # graph replays); the convergence stop halts the rest on the device
-        srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=200)
+        srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=8, chunk=200, max_group=SERVE_GROUP,
+                           max_group_nodes=SERVE_GROUP_NODES)
         ids, done = [], {}
         t0 = time.perf_counter()
         k = 0
@@ -499,6 +503,7 @@ def bench_serving(rt, n_req, rates, seed=50):
     return dict(requests=n_req, nodes_min=int(sizes.min()), nodes_max=int(sizes.max()),
                 solves_per_s=sat["solves_per_s"], p50_latency_s=sat["p50_latency_s"],
                 p99_latency_s=sat["p99_latency_s"], saturated_at=sat["arrival_rate_per_s"], sweep=sweep,
+                grouping=dict(max_group=SERVE_GROUP, max_group_nodes=SERVE_GROUP_NODES),
                 status_ok=all(x["status_ok"] for x in sweep))
 
 
