@@ -228,7 +228,7 @@ def batch64_alone(rt, batch64):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("variant", ["mixed_coloured", "part_coloured"])
+@pytest.mark.parametrize("variant", ["mixed_coloured", "mixed_atomic", "part_coloured"])
 def test_batch64_each_mesh_vs_its_own_solve(rt, batch64, variant):
     meshes, mach, aoa = batch64
     alone = batch64_alone(rt, batch64)
