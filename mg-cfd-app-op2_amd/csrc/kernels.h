@@ -205,10 +205,21 @@ __device__ __forceinline__ int xcd_block()
     const int nb = gridDim.x, b = blockIdx.x;
     const int per = nb / 8, rem = nb % 8;
     const int x = b % 8, k = b / 8;
-    return (x < rem) ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+    // XCD x's slice starts at x * per + min(x, rem) (the first rem slices
+    // hold one more); no branch, so the kernel's first loads issue together
+    return x * per + min(x, rem) + k;
 }
 
 __device__ __forceinline__ bool halted(const int* status) { return status && *status != 0; }
+
+// base[i], or 0 when base is null, read by one load with no branch around it
+// (a null base reads a zero word).  Kernels that test their context's status
+// late (before their first global write, not at entry) read it this way with
+// their first loads, so its round trip overlaps theirs instead of preceding
+// them.
+__device__ int kZeroWord = 0;     // never written
+__device__ __forceinline__ int word_or_zero(const int* base, int i) { return *(base ? base + i : &kZeroWord); }
+__device__ __forceinline__ int status_of(const int* status) { return word_or_zero(status, 0); }
 
 // ------------------------------------------------------------------------
 // Edge flux, blocked gather (default).  Workgroup b owns the nodes
@@ -727,11 +738,12 @@ __global__ __launch_bounds__(T) void k_flux_part(
     double* __restrict__ flux, Consts k, const int* __restrict__ status, int fresh)
 {
     extern __shared__ double sm[];
-    if (halted(status)) return;
+    const int halt = status_of(status);     // tested before the store
     const PartInfo P = parts[xcd_block()];
     const double* acc =
         part_accumulate<R, T, EPT, NPT, COLOURED, RS, false>(P, halo, ecode, ew, nloc, rec_cap, var, flux, k, fresh,
                                                              sm);
+    if (halt) return;
     // 4. store
     double* fw = flux + (size_t)P.n0 * 5;
     for (int j = threadIdx.x; j < P.cnt * 5; j += T) fw[j] = acc[j];
@@ -1405,16 +1417,19 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a, Consts k0)
 // different buffers.  The edge flux starts from the flux dat on the first
 // stage (fresh == 0) and from zero after it; it is not written back.
 // pseg / pfirst: segment of each part and first part of each segment
-// (batched solves), null for one mesh.  Returns false when the context
-// aborts (negative minimum dt).
+// (batched solves), null for one mesh.  Returns false when the context has
+// halted or aborts (negative minimum dt).
 template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
 __device__ __forceinline__ bool part_stage_one(
     int p, const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
     const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
     const StageArgs& a, const Consts& k0, const int* __restrict__ pseg, const int* __restrict__ pfirst, double* sm)
 {
+    // a halted context writes nothing: the status is tested after the edge
+    // phase (which writes only LDS), so its load overlaps the staging loads
+    const int halt = status_of(a.status);
     const PartInfo P = parts[p];
-    const int seg = pseg ? pseg[p] : 0;
+    const int seg = word_or_zero(pseg, p);
     static_assert(T >= kPart, "one owned node per thread");
     // the owned node's old state and volume / step factor, in flight through
     // the edge phase (clamped to a valid node for threads without one)
@@ -1426,6 +1441,7 @@ __device__ __forceinline__ bool part_stage_one(
     const int q0 = a.bptr ? a.bptr[i] : 0, q1 = a.bptr ? a.bptr[i + 1] : 0;
     double* acc = part_accumulate<R, T, EPT, NPT, COLOURED, RS, true>(P, halo, ecode, ew, nloc, rec_cap, vin,
                                                                       a.flux, k0, fresh, sm);
+    if (halt) return false;
     // the minimum dt after the edge phase (which writes only LDS), before any
     // update: fewer live registers through the edge phase
     double mdt = 0.0;
@@ -1448,12 +1464,11 @@ __device__ __forceinline__ bool part_stage_one(
 // One workgroup per part.
 template <typename R, int T, int EPT, int NPT, bool COLOURED, int RS>
 __global__ __launch_bounds__(T) void k_flux_part_stage(
-    const PartInfo* __restrict__ parts, const int* __restrict__ halo, const unsigned* __restrict__ ecode,
-    const R* __restrict__ ew, long long nloc, int rec_cap, const double* __restrict__ vin, int fresh,
-    StageArgs a, Consts k0, const int* __restrict__ pseg, const int* __restrict__ pfirst)
+    const PartInfo* __restrict__ parts, const int* __restrict__ pseg, const int* __restrict__ pfirst,
+    const int* __restrict__ halo, const unsigned* __restrict__ ecode, const R* __restrict__ ew, long long nloc,
+    int rec_cap, const double* __restrict__ vin, int fresh, StageArgs a, Consts k0)
 {
     extern __shared__ double sm[];
-    if (halted(a.status)) return;
     if (part_stage_one<R, T, EPT, NPT, COLOURED, RS>(xcd_block(), parts, halo, ecode, ew, nloc, rec_cap, vin, fresh,
                                                      a, k0, pseg, pfirst, sm))
         stage_reset_min<T>(a);

@@ -1405,12 +1405,12 @@ static int launch_part_rs(bool coloured, const InvPlan& P, const Real* w, size_t
         }
         if (coloured)
             k_flux_part_stage<Real, T, EPT, NPT, true, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
-                P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
-                st->pseg, st->pfirst);
+                P.d_pt_info, st->pseg, st->pfirst, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin,
+                fresh, *st->a, R.k);
         else
             k_flux_part_stage<Real, T, EPT, NPT, false, RS><<<P.pt_parts, T, lds, cur_stream()>>>(
-                P.d_pt_info, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin, fresh, *st->a, R.k,
-                st->pseg, st->pfirst);
+                P.d_pt_info, st->pseg, st->pfirst, P.d_pt_halo, P.d_pt_code, w, P.pt_nloc, P.pt_rec_cap, st->vin,
+                fresh, *st->a, R.k);
         return check_launch();
     }
     if (coloured)
