@@ -388,9 +388,11 @@ int mgcfd_batch_destroy(mgcfd_batch b);
  * whose mesh segments are its members, on its own stream, so one launch per
  * loop serves them all; its cycles are issued as plain launches in chunks of
  * cycles_per_chunk.  Groups that arrive while others run overlap with them
- * on the device.  A request's result is the same alone and in any mix: a
- * member that converges has its state copied at that cycle, and one that
- * aborts (negative dt, NaN) is served again alone.
+ * on the device.  A request's result is the same alone and in any mix
+ * (bitwise with the deterministic variants: exact gathers and coloured part
+ * kernels; within rounding with the atomic ones): a member that converges
+ * has its state copied at that cycle, and one that aborts (negative dt, NaN)
+ * is served again alone.
  * The caller drives the loop with mgcfd_server_progress (non-blocking).
  */
 typedef struct mgcfd_server_s* mgcfd_server;
