@@ -1551,21 +1551,50 @@ __global__ __launch_bounds__(kBlock) void k_sumsq_part(int n, const double* __re
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// Final reductions of per-workgroup partials run as one workgroup per sum.
+// Each thread keeps four partial sums over entries NT apart, so four loads
+// are in flight per thread instead of one (a level-0 sum of an 8M-node mesh
+// has 31K partials); the order is fixed, so the sum is deterministic.
+constexpr int kSumBlock = 1024;
+
+template <int NT>
+__device__ __forceinline__ double strided_sum(const double* __restrict__ x, const int* __restrict__ cnt, int lo,
+                                              int hi, int& nb)
+{
+#pragma clang fp contract(off)
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = 0;
+    int i = lo + (int)threadIdx.x;
+    for (; i + 3 * NT < hi; i += 4 * NT) {
+        s0 += x[i];
+        s1 += x[i + NT];
+        s2 += x[i + 2 * NT];
+        s3 += x[i + 3 * NT];
+        if (cnt) c += cnt[i] + cnt[i + NT] + cnt[i + 2 * NT] + cnt[i + 3 * NT];
+    }
+    for (; i < hi; i += NT) {
+        s0 += x[i];
+        if (cnt) c += cnt[i];
+    }
+    nb = c;
+    return (s0 + s1) + (s2 + s3);
+}
+
 // Sum of partials; *out = init + sum.  With 'hist' the L0 RMS
 // sqrt(sum / n) is stored at hist[*cycle] (euler3d.cpp:532-537) and a
 // non-zero bad-value count raises abort code 2 (euler3d.cpp:539-548).
-__global__ __launch_bounds__(kBlock) void k_sum_final(int np, const double* __restrict__ part,
-                                                       double init, double* __restrict__ out,
-                                                       int nnodes, double* __restrict__ hist,
-                                                       const int* __restrict__ cycle,
-                                                       const int* __restrict__ bad,
-                                                       int* __restrict__ status)
+__global__ __launch_bounds__(kSumBlock) void k_sum_final(int np, const double* __restrict__ part,
+                                                          double init, double* __restrict__ out,
+                                                          int nnodes, double* __restrict__ hist,
+                                                          const int* __restrict__ cycle,
+                                                          const int* __restrict__ bad,
+                                                          int* __restrict__ status)
 {
 #pragma clang fp contract(off)
     if (status && *status != 0) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
-    s = block_sum(s);
+    int unused = 0;
+    double s = strided_sum<kSumBlock>(part, nullptr, 0, np, unused);
+    s = block_sum<kSumBlock>(s);
     if (threadIdx.x == 0) {
         s = init + s;
         if (out) *out = s;
@@ -1968,23 +1997,20 @@ __global__ __launch_bounds__(kBlock) void k_calc_dt_seg(const int4* __restrict__
     min_publish(pub, B.x, block_min(d));
 }
 
-__global__ __launch_bounds__(kBlock) void k_sum_final_seg(const int* __restrict__ sblk, const double* __restrict__ part,
-                                                          const int* __restrict__ badpart,
-                                                          const int* __restrict__ nnodes, double* __restrict__ hist,
-                                                          int cap, const int* __restrict__ cycle,
-                                                          int* __restrict__ sstatus, const int* __restrict__ halt)
+__global__ __launch_bounds__(kSumBlock) void k_sum_final_seg(const int* __restrict__ sblk,
+                                                             const double* __restrict__ part,
+                                                             const int* __restrict__ badpart,
+                                                             const int* __restrict__ nnodes, double* __restrict__ hist,
+                                                             int cap, const int* __restrict__ cycle,
+                                                             int* __restrict__ sstatus, const int* __restrict__ halt)
 {
 #pragma clang fp contract(off)
     if (halted(halt)) return;
     const int s = blockIdx.x;
-    double x = 0.0;
     int nb = 0;
-    for (int i = sblk[s] + threadIdx.x; i < sblk[s + 1]; i += kBlock) {
-        x += part[i];
-        nb += badpart[i];
-    }
-    x = block_sum(x);
-    nb = block_sum_i(nb);
+    double x = strided_sum<kSumBlock>(part, badpart, sblk[s], sblk[s + 1], nb);
+    x = block_sum<kSumBlock>(x);
+    nb = block_sum_i<kSumBlock>(nb);
     if (threadIdx.x == 0) {
         const int c = *cycle;
         if (c < cap) hist[(size_t)s * cap + c] = sqrt(x / (double)nnodes[s]);

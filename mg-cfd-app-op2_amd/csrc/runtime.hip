@@ -2292,7 +2292,7 @@ int op_par_loop_calc_rms_kernel(const char* name, op_set set, op_arg a0, op_arg 
     const int nb = (int)nblk(n);
     TRY(ensure_part(nb));
     k_sumsq_part<<<nb, kBlock, 0, cur_stream()>>>(n, dptr(a0.dat), R.d_part);
-    k_sum_final<<<1, kBlock, 0, cur_stream()>>>(nb, R.d_part, init, R.d_scal + 1, n, nullptr, nullptr, nullptr, nullptr);
+    k_sum_final<<<1, kSumBlock, 0, cur_stream()>>>(nb, R.d_part, init, R.d_scal + 1, n, nullptr, nullptr, nullptr, nullptr);
     TRY(check_launch());
     double s;
     TRY(read_scalar(R.d_scal + 1, &s));
