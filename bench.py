@@ -27,6 +27,17 @@ also launches:
   rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py --steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-shard --no-solve --no-large
   python3 bench.py --summarize-pmc OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
 
+The 8M-node solve's HBM bytes per multigrid cycle come the same way, from a
+kernel-trace run and two counter runs of the solve alone (ARGS = --steps 10
+--warmup 2 --no-cpu --no-batch --no-serve --no-shard --no-solve
+--large-cycles 5), summarised into profiles/mg_cycle_traffic_8m.json; the
+large leg divides them by its own time per cycle:
+
+  rocprofv3 --kernel-trace -f csv -d OUT/trace -o run -- python3 bench.py ARGS
+  rocprofv3 --pmc FETCH_SIZE -f csv -d OUT/fetch -o run -- python3 bench.py ARGS
+  rocprofv3 --pmc WRITE_SIZE -f csv -d OUT/write -o run -- python3 bench.py ARGS
+  python3 bench.py --summarize-large OUT/trace/run_kernel_trace.csv OUT/fetch/run_counter_collection.csv OUT/write/run_counter_collection.csv
+
 Beside the headline the JSON line carries: the unstructured-stream loop on
 the same copies (the access pattern's ceiling, which the reference measures
 beside the flux under -b), the flux at a 1M-node shard shape, a 50-cycle
@@ -58,6 +69,7 @@ FLOP_PER_EDGE = 201             # flux.h:48-207 as written: 186 add/mul + 8 div 
 N_NODES, N_EDGES = 300_000, 930_000
 MACH, AOA = 0.5, 3.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "flux_traffic.json")
+PROFILE_LARGE_TRAFFIC = os.path.join(ROOT, "profiles", "mg_cycle_traffic_8m.json")
 # served requests queued behind a busy server share a context in groups of at
 # most this many requests and level-0 nodes
 SERVE_GROUP, SERVE_GROUP_NODES = 8, 4_000_000
@@ -250,6 +262,11 @@ def bench_large(rt, cycles=20, n=8_000_000, mach=0.7, aoa=2.0):
         b.close()
         out[variant] = dict(seconds=round(el, 3), ms_per_cycle=round(el / cycles * 1e3, 3),
                             status_ok=bool((st == 0).all()), setup_seconds=round(setup, 1))
+        traffic = large_traffic(variant)
+        if traffic:
+            gbs = traffic / (el / cycles) / 1e9
+            out[variant].update(hbm_bytes_per_cycle=traffic, hbm_gbs=round(gbs, 1),
+                                hbm_frac=round(gbs / HBM_PEAK_GBS, 4))
         if ref is None:
             ref = u
         else:
@@ -259,7 +276,22 @@ def bench_large(rt, cycles=20, n=8_000_000, mach=0.7, aoa=2.0):
         log(f"bench: large: {variant}: {out[variant]}")
     del mesh
     out["speedup"] = round(out["part_atomic"]["seconds"] / out["mixed_atomic"]["seconds"], 3)
+    if large_traffic("part_atomic"):
+        out["hbm_traffic_source"] = (f"{os.path.relpath(PROFILE_LARGE_TRAFFIC, ROOT)}: HBM bytes per cycle from "
+                                     "rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script (scaled by what they "
+                                     "read for k_stream_copy of 512 MiB), not measured in this run; divided by "
+                                     "this run's time per cycle")
     return out
+
+
+def large_traffic(variant):
+    """HBM bytes per multigrid cycle of the 8M-node solve, from the committed
+    counter summary (None when absent)."""
+    try:
+        with open(PROFILE_LARGE_TRAFFIC) as fh:
+            return json.load(fh)[variant]["bytes_per_cycle"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def check_variant(rt, solver, mesh0, variant, orc):
@@ -365,6 +397,89 @@ def summarize_pmc(fetch_csv, write_csv):
     with open(PROFILE_TRAFFIC, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
+
+
+def pmc_dispatches(path):
+    """(dispatch id, kernel name, grid size in workgroups, counter KiB) of a
+    rocprofv3 counter CSV, in dispatch order."""
+    acc = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            d = int(row["Dispatch_Id"])
+            g = int(row.get("Grid_Size") or row.get("Grid_Size_X") or 0)
+            wg = int(row.get("Workgroup_Size") or row.get("Workgroup_Size_X") or 1)
+            name, _, _, v = acc.get(d, (row["Kernel_Name"], g // max(wg, 1), 0, 0.0))
+            acc[d] = (name, g // max(wg, 1), 0, v + float(row["Counter_Value"]))
+    return [(d, acc[d][0], acc[d][1], acc[d][3]) for d in sorted(acc)]
+
+
+def cycle_windows(disp, pat):
+    """The dispatches of the full multigrid cycles of one solve: the cycles
+    end with k_bump; those after the first end that follows the first
+    dispatch matching 'pat', up to the first end after its last one."""
+    hits = [i for i, x in enumerate(disp) if re.search(pat, x[1])]
+    if not hits:
+        return None, 0
+    ends = [i for i, x in enumerate(disp) if "k_bump" in x[1] and i > hits[0]]
+    last = next(i for i in ends if i > hits[-1])
+    ends = [i for i in ends if i <= last]
+    return disp[ends[0] + 1:last + 1], len(ends) - 1
+
+
+def summarize_large(trace_csv, fetch_csv, write_csv):
+    """Per-cycle HBM traffic and kernel time of the 8M-node solve (part_atomic
+    and mixed_atomic) from a kernel-trace run and two counter runs of
+    'bench.py --no-cpu --no-batch --no-serve --no-shard --no-solve
+    --large-cycles 5'; writes profiles/mg_cycle_traffic_8m.json and prints a
+    markdown table."""
+    fetch, write = pmc_dispatches(fetch_csv), pmc_dispatches(write_csv)
+    cf = [v for _, k, _, v in fetch if "k_stream_copy" in k]
+    cw = [v for _, k, _, v in write if "k_stream_copy" in k]
+    if not cf or not cw:
+        sys.exit("summarize-large: no k_stream_copy dispatch in the counter files")
+    sf, sw = COPY_BYTES / (np.mean(cf) * 1024), COPY_BYTES / (np.mean(cw) * 1024)
+    trace = []
+    with open(trace_csv, newline="") as f:
+        for row in csv.DictReader(f):
+            g = int(row.get("Grid_Size_X") or row.get("Grid_Size") or 0)
+            wg = int(row.get("Workgroup_Size_X") or row.get("Workgroup_Size") or 1)
+            trace.append((int(row["Dispatch_Id"]), row["Kernel_Name"], g // max(wg, 1),
+                          (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) * 1e-3))
+    trace.sort()
+    out = {"copy_kernel": {"fetch_scale": round(sf, 4), "write_scale": round(sw, 4)}}
+    rows = []
+    for variant, pat in (("part_atomic", r"k_flux_part_stage<double"), ("mixed_atomic", r"k_flux_part_stage<float")):
+        fw, nf = cycle_windows(fetch, pat)
+        ww, nw = cycle_windows(write, pat)
+        tw, nt = cycle_windows(trace, pat)
+        if not fw or not ww or not tw or not (nf == nw == nt) or len(fw) != len(tw):
+            sys.exit(f"summarize-large: the {variant} cycles do not line up in the three runs")
+        per = {}
+        for (_, k, g, f), (_, _, _, w), (_, _, _, us) in zip(fw, ww, tw):
+            key = (re.sub(r"\(.*", "", k).replace("mgcfd::", ""), g)
+            a = per.setdefault(key, [0, 0.0, 0.0])
+            a[0] += 1
+            a[1] += f * 1024 * sf + w * 1024 * sw
+            a[2] += us
+        total_b = sum(a[1] for a in per.values()) / nf
+        total_us = sum(a[2] for a in per.values()) / nf
+        out[variant] = {"cycles": nf, "bytes_per_cycle": round(total_b), "kernel_us_per_cycle": round(total_us, 1),
+                        "kernels": [{"kernel": k, "workgroups": g, "dispatches_per_cycle": a[0] / nf,
+                                     "bytes_per_cycle": round(a[1] / nf), "us_per_cycle": round(a[2] / nf, 1)}
+                                    for (k, g), a in sorted(per.items(), key=lambda kv: -kv[1][2])]}
+        rows.append(f"\n{variant}: {nf} cycles, {total_b / 1e9:.2f} GB and {total_us:.0f} us of kernels per cycle, "
+                    f"{total_b / (total_us * 1e-6) / 1e9:.0f} GB/s\n")
+        rows.append("| kernel | workgroups | per cycle | µs per cycle | GB per cycle | GB/s |")
+        rows.append("|---|---|---|---|---|---|")
+        for e in out[variant]["kernels"]:
+            if e["us_per_cycle"] < 1.0:
+                continue
+            rows.append(f"| `{e['kernel']}` | {e['workgroups']} | {e['dispatches_per_cycle']:g} | {e['us_per_cycle']} | "
+                        f"{e['bytes_per_cycle'] / 1e9:.3f} | {e['bytes_per_cycle'] / (e['us_per_cycle'] * 1e-6) / 1e9:.0f} |")
+    os.makedirs(os.path.dirname(PROFILE_LARGE_TRAFFIC), exist_ok=True)
+    with open(PROFILE_LARGE_TRAFFIC, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("\n".join(rows))
 
 
 def bench_batched(rt, n_meshes, cycles, fp64_variant, seed=40):
@@ -527,11 +642,18 @@ def main():
     ap.add_argument("--no-shard", action="store_true", help="skip the 1M-node shard-shape flux measurement")
     ap.add_argument("--no-solve", action="store_true", help="skip the 300K-node 50-cycle solve timing")
     ap.add_argument("--no-large", action="store_true", help="skip the 8M-node solve (mixed precision against FP64)")
+    ap.add_argument("--large-cycles", type=int, default=20, help="timed cycles of the 8M-node solve")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
+    ap.add_argument("--summarize-large", nargs=3, metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/mg_cycle_traffic_8m.json from a kernel-trace run and two counter runs "
+                         "of this script's 8M-node solve")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_large:
+        summarize_large(*args.summarize_large)
         return
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
@@ -641,7 +763,7 @@ def main():
     batched = None if args.no_batch else bench_batched(rt, args.batch_meshes, args.batch_cycles, fp64_solver)
     serving = None if args.no_serve else bench_serving(rt, args.serve_requests,
                                                            [float(x) for x in args.serve_rates.split(",")])
-    large = None if args.no_large else bench_large(rt)
+    large = None if args.no_large else bench_large(rt, cycles=args.large_cycles)
     rt.close()
 
     out = {
