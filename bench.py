@@ -11,7 +11,8 @@ beside it.
 Roofline: algorithmic bytes per call B = 32 E + 120 N (24 B weights + 8 B map
 per edge; 40 B variables read, 40 B flux read, 40 B flux write per node)
 against the 8 TB/s HBM peak.  Kernel time comes from HIP events on the
-library's stream.
+library's stream around one graph replay of the timed calls (the device runs
+them back to back, with no host work between them).
 
 The CPU baseline is the reference's own flux kernel (flux.h compiled in
 place by oracle/Makefile into oracle/_ref), run OpenMP-parallel over

@@ -311,11 +311,12 @@ int mgcfd_mg_bind(int levels, const mgcfd_level_bind* binds);
  * changing at the failing smoothing step.  Returns 0 or a negative code. */
 int mgcfd_mg_run(int cycles, int use_graph, double* rms_hist, int* status);
 
-/* Times 'iters' back-to-back launches of the edge-flux loop on the bound
- * level with HIP events on the library stream; *ms_per_call receives the
- * mean.  'dats' (n_copies >= 1) rotate the variables/fluxes/weights among
- * independent copies so that the working set can exceed the last-level
- * cache. */
+/* Times 'iters' back-to-back launches of the edge-flux loop with HIP events
+ * on the library stream; *ms_per_call receives the mean.  The launches are
+ * captured into one graph and replayed once between the events, so the
+ * host's per-launch work does not leave the device idle between them.  The
+ * copies (n_copies >= 1 maps with their variables, weights and fluxes)
+ * rotate so that the working set can exceed the last-level cache. */
 int mgcfd_time_flux(int variant, int n_copies, const op_map* edge_maps, const op_dat* vars,
                     const op_dat* weights, const op_dat* fluxes, int iters, double* ms_per_call);
 

@@ -211,6 +211,43 @@ def test_time_flux_calls_entry(rt, orc, knn_small):
     assert each[0] > 0 and each[1] > 0
 
 
+@pytest.mark.parametrize("variant", ["gather_exact", "part_atomic"])
+def test_time_flux_entry_replays_each_call(rt, orc, knn_small, variant):
+    """mgcfd_time_flux (the bench's headline timing) captures its calls into
+    one graph and replays it: three calls rotating over two copies add two
+    flux passes to the first copy and one to the second, as three sequential
+    oracle passes do (bitwise for the exact variant)."""
+    c = orc.freestream(0.5, 3.0)
+    sols = [mgcfd.Solver(rt, knn_small, 0.5, 3.0) for _ in range(2)]
+    rt.set_flux_variant(mgcfd.FLUX_VARIANTS[variant])
+    for s in sols:
+        s.setup()
+    ref = oracle_setup(orc, knn_small, c)
+    m0 = knn_small[0]
+    n = len(m0["coords"])
+    e2n = np.ascontiguousarray(m0["edge_to_node"], np.int32)
+    vars_ = [perturbed_state(n, c.ff_variable[:], seed=36 + k) for k in range(2)]
+    for s, v in zip(sols, vars_):
+        rt.upload(s.lv[0].variables, v)
+        rt.upload(s.lv[0].fluxes, np.zeros((n, 5)))
+    args = ((mgcfd.OpMap * 2)(*[s.lv[0].edge_to_node for s in sols]),
+            (mgcfd.OpDat * 2)(*[s.lv[0].variables for s in sols]),
+            (mgcfd.OpDat * 2)(*[s.lv[0].edge_weights for s in sols]),
+            (mgcfd.OpDat * 2)(*[s.lv[0].fluxes for s in sols]))
+    ms = C.c_double(0)
+    assert rt.L.mgcfd_time_flux(mgcfd.FLUX_VARIANTS[variant], 2, *args, 3, C.byref(ms)) == 0
+    assert ms.value > 0.0
+    for k, passes in ((0, 2), (1, 1)):
+        o = np.zeros_like(vars_[k])
+        for _ in range(passes):
+            orc.compute_flux_edge(e2n, vars_[k], ref[0]["ew"], o, c)
+        g = sols[k].state(0, "fluxes")
+        if variant == "gather_exact":
+            assert bitwise(g, o), f"copy {k}"
+        else:
+            assert rel_err(g, o) <= FLUX_FAST_TOL, f"copy {k}"
+
+
 def test_time_stepping_loops(rt, orc, knn_small):
     """copy, calculate_dt, get_min_dt, step factor, time_step, residual, rms, bad values."""
     c = orc.freestream(0.5, 3.0)
