@@ -107,10 +107,10 @@ def test_request_over_budget_refused(rt, meshes):
 
 
 def test_admitted_requests_overlap_on_device(rt, meshes):
-    """Requests admitted together run concurrently: their device intervals
-    (HIP events on each request's stream) intersect."""
+    """Requests in contexts of their own (no grouping) run concurrently:
+    their device intervals (HIP events on each context's stream) intersect."""
     big = [mgcfd.mesh_knn(20000, 62000, 3, seed=16 + k) for k in range(3)]
-    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=4, chunk=400)
+    srv = mgcfd.Server(rt, budget_bytes=200e9, max_active=4, chunk=400, max_group=1)
     # all three are queued before the first progress call admits them
     ids = [srv.submit(m, 0.5, 3.0, "part_coloured", max_cycles=400) for m in big]
     res = drain(srv, ids, max_active=4)
