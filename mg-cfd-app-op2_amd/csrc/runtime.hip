@@ -856,19 +856,17 @@ struct PartOut {
 // reads of 48-byte records (whose bank is fixed by the slot modulo 8), 16
 // per pass for 8-byte accesses (accumulators at a 40-byte stride and
 // 40-byte records: slot modulo 16).  Each position takes, among the next
-// candidates of its colour, the edge whose endpoint slots collide least
-// with those already in its 8- and 16-lane groups.
-template <typename LEv, typename SlotFn>
-static void bank_order(std::vector<LEv>& le, SlotFn slot)
+// candidates of its colour, the edge whose endpoint slots (sa, sb) collide
+// least with those already in its 8- and 16-lane groups.  The result is
+// left in 'le'; 'out' is scratch space.
+template <typename LEv>
+static void bank_order(std::vector<LEv>& le, std::vector<LEv>& out)
 {
     constexpr size_t kLook = 48;
-    std::vector<LEv> out;
+    out.clear();
     out.reserve(le.size());
-    std::vector<int> sa(le.size()), sb(le.size());
-    for (size_t i = 0; i < le.size(); i++) {
-        sa[i] = slot(le[i].a);
-        sb[i] = slot(le[i].b);
-    }
+    auto sa = [&](size_t i) { return le[i].sa; };
+    auto sb = [&](size_t i) { return le[i].sb; };
     std::vector<size_t> rest;
     unsigned m8a = 0, m8b = 0, m16a = 0, m16b = 0;
     for (size_t lo = 0; lo < le.size();) {
@@ -885,8 +883,8 @@ static void bank_order(std::vector<LEv>& le, SlotFn slot)
             int bc = 1 << 30;
             for (size_t k = head; k < rest.size() && k < head + kLook; k++) {
                 const size_t i = rest[k];
-                const int c = ((m8a >> (sa[i] & 7)) & 1) + ((m8b >> (sb[i] & 7)) & 1) +
-                              ((m16a >> (sa[i] & 15)) & 1) + ((m16b >> (sb[i] & 15)) & 1);
+                const int c = ((m8a >> (sa(i) & 7)) & 1) + ((m8b >> (sb(i) & 7)) & 1) +
+                              ((m16a >> (sa(i) & 15)) & 1) + ((m16b >> (sb(i) & 15)) & 1);
                 if (c < bc) {
                     bc = c;
                     best = k;
@@ -895,10 +893,10 @@ static void bank_order(std::vector<LEv>& le, SlotFn slot)
             }
             std::swap(rest[head], rest[best]);
             const size_t i = rest[head++];
-            m8a |= 1u << (sa[i] & 7);
-            m8b |= 1u << (sb[i] & 7);
-            m16a |= 1u << (sa[i] & 15);
-            m16b |= 1u << (sb[i] & 15);
+            m8a |= 1u << (sa(i) & 7);
+            m8b |= 1u << (sb(i) & 7);
+            m16a |= 1u << (sa(i) & 15);
+            m16b |= 1u << (sb(i) & 15);
             out.push_back(le[i]);
         }
         lo = hi;
@@ -912,8 +910,10 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
     std::vector<int2> work{make_int2(lo0, hi0)};
     std::vector<int> hb;
     std::vector<unsigned long long> used(kPart);   // per owned node: colours taken
-    struct LE { int a, b, e; unsigned flags; int col; };
-    std::vector<LE> le;
+    // a local edge: endpoints, their LDS slots, edge id, owner flags, colour
+    struct LE { int a, b, sa, sb, e; unsigned flags; int col; };
+    std::vector<LE> le, tmp;
+    std::vector<int> ccount;
     while (!work.empty()) {
         const int lo = work.back().x, hi = work.back().y;
         work.pop_back();
@@ -935,6 +935,7 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
             work.push_back(make_int2(lo, mid));
             continue;
         }
+        // owned nodes take slots [0, hi - lo), the sorted halo the rest
         auto slot = [&](int nd) -> int {
             if (nd >= lo && nd < hi) return nd - lo;
             return (hi - lo) + (int)(std::lower_bound(hb.begin(), hb.end(), nd) - hb.begin());
@@ -949,8 +950,9 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
                 if (other_owned && role_b) continue;     // interior edge: taken from its a side
                 LE x;
                 x.e = e;
-                if (!role_b) { x.a = nd; x.b = o; x.flags = 1u | (other_owned ? 2u : 0u); }
-                else { x.a = o; x.b = nd; x.flags = 2u; }
+                const int so = slot(o);
+                if (!role_b) { x.a = nd; x.b = o; x.sa = nd - lo; x.sb = so; x.flags = 1u | (other_owned ? 2u : 0u); }
+                else { x.a = o; x.b = nd; x.sa = so; x.sb = nd - lo; x.flags = 2u; }
                 le.push_back(x);
             }
         // greedy colouring in edge order
@@ -971,8 +973,14 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
             if (x.flags & 2u) used[x.b - lo] |= 1ull << c;
             ncol = std::max(ncol, c + 1);
         }
-        std::stable_sort(le.begin(), le.end(), [](const LE& x, const LE& y) { return x.col < y.col; });
-        bank_order(le, [&](int nd) { return slot(nd); });
+        // stable counting sort by colour (edge order kept within a colour)
+        ccount.assign(ncol + 1, 0);
+        for (const auto& x : le) ccount[x.col + 1]++;
+        for (int c = 0; c < ncol; c++) ccount[c + 1] += ccount[c];
+        tmp.resize(le.size());
+        for (const auto& x : le) tmp[ccount[x.col]++] = x;
+        le.swap(tmp);
+        bank_order(le, tmp);
         PartInfo pi;
         pi.n0 = lo;
         pi.cnt = hi - lo;
@@ -983,8 +991,7 @@ static void part_part(int lo0, int hi0, int psize, const std::vector<int>& ptr, 
         pi.ncol = ncol;
         pi.pad = 0;
         for (const auto& x : le) {
-            const unsigned sa = (unsigned)slot(x.a), sb = (unsigned)slot(x.b);
-            out.code.push_back(sa | (sb << 10) | (x.flags << 20) | ((unsigned)x.col << 22));
+            out.code.push_back((unsigned)x.sa | ((unsigned)x.sb << 10) | (x.flags << 20) | ((unsigned)x.col << 22));
             out.eid.push_back(x.e);
         }
         out.halo.insert(out.halo.end(), hb.begin(), hb.end());
