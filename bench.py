@@ -483,6 +483,70 @@ def summarize_large(trace_csv, fetch_csv, write_csv):
     print("\n".join(rows))
 
 
+def summarize_trace(stats_or_trace, mode, pattern=None):
+    """Markdown tables from rocprofv3 CSVs for the committed profile summaries:
+    mode 'stats': the kernel statistics file (run_kernel_stats.csv), one row
+    per kernel; mode 'cycle': the multigrid-cycle window of a kernel trace
+    (run_kernel_trace.csv) from the first to the last dispatch matching
+    'pattern' (default: the FP64 PART_ATOMIC fused stage kernel), grouped by
+    kernel and grid size, per cycle (18 stage kernels per cycle); mode
+    'bursts': the dispatches matching 'pattern' in launch order, grouped into
+    bursts (a new burst after more than 20 us with none running), with each
+    burst's mean kernel time, start-to-start interval and gap."""
+    def short(n, k=90):
+        n = n.replace("mgcfd::", "")
+        return n if len(n) <= k else n[:k] + "…"
+    with open(stats_or_trace, newline="") as f:
+        rows = list(csv.DictReader(f))
+    out = []
+    if mode == "stats":
+        out.append("| kernel | calls | avg µs | min µs | max µs | total % |\n|---|---|---|---|---|---|")
+        for r in rows[:21]:
+            out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                       f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+    elif mode == "cycle":
+        pat = pattern or r"k_flux_part_stage<double, 256, \d+, \d+, false"
+        rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+        idx = [i for i, r in enumerate(rows) if re.search(pat, r["Kernel_Name"])]
+        win = rows[idx[0]:idx[-1] + 1]
+        nst = sum(1 for r in win if re.search(pat, r["Kernel_Name"]))
+        cycles = nst / 18
+        span = (int(win[-1]["End_Timestamp"]) - int(win[0]["Start_Timestamp"])) / 1e3
+        busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in win) / 1e3
+        g = {}
+        for r in win:
+            wg = int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"])
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).replace("mgcfd::", "")
+            g.setdefault((name, wg), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        out.append(f"Window: {len(win)} dispatches, {nst} fused stage kernels ({cycles:.0f} cycles), {span:.0f} µs "
+                   f"from the first start to the last end, {busy:.0f} µs of kernel time.")
+        out.append(f"Per cycle: {span / cycles:.1f} µs span, {busy / cycles:.1f} µs of kernels, "
+                   f"{len(win) / cycles:.0f} dispatches.\n")
+        out.append("| kernel | workgroups | dispatches per cycle | avg µs | µs per cycle |\n|---|---|---|---|---|")
+        for (name, wg), d in sorted(g.items(), key=lambda kv: -sum(kv[1])):
+            out.append(f"| `{name}` | {wg} | {len(d) / cycles:.1f} | {sum(d) / len(d):.2f} | {sum(d) / cycles:.1f} |")
+    elif mode == "bursts":
+        pat = pattern or VARIANT_KERNELS["part_atomic"]
+        ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows
+                    if re.search(pat, r["Kernel_Name"]))
+        bursts, cur = [], [ks[0]]
+        for a, b in ks[1:]:
+            if a - cur[-1][1] > 20000:
+                bursts.append(cur)
+                cur = []
+            cur.append((a, b))
+        bursts.append(cur)
+        out.append("| burst | launches | mean kernel µs | mean start-to-start µs | mean gap µs |\n|---|---|---|---|---|")
+        for i, bu in enumerate(bursts):
+            d = sum(b - a for a, b in bu) / len(bu) / 1e3
+            s2s = (bu[-1][0] - bu[0][0]) / max(len(bu) - 1, 1) / 1e3
+            gap = sum(bu[j + 1][0] - bu[j][1] for j in range(len(bu) - 1)) / max(len(bu) - 1, 1) / 1e3
+            out.append(f"| {i} | {len(bu)} | {d:.2f} | {s2s:.2f} | {gap:.2f} |")
+    else:
+        sys.exit(f"summarize-trace: unknown mode {mode}")
+    print("\n".join(out))
+
+
 def bench_batched(rt, n_meshes, cycles, fp64_variant, seed=40):
     """64 independent solves (N log-uniform in [20K, 500K], E = 3.1 N, 4
     levels, Mach U(0.3, 0.8), angle U(-2, 6) deg), 30 cycles each:
@@ -646,6 +710,9 @@ def main():
     ap.add_argument("--large-cycles", type=int, default=20, help="timed cycles of the 8M-node solve")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
+    ap.add_argument("--summarize-trace", nargs="+", metavar="ARG",
+                    help="CSV MODE [PATTERN]: markdown tables of a rocprofv3 kernel-statistics file (MODE stats) or "
+                         "kernel trace (MODE cycle or bursts) for the profile summaries")
     ap.add_argument("--summarize-large", nargs=3, metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/mg_cycle_traffic_8m.json from a kernel-trace run and two counter runs "
                          "of this script's 8M-node solve")
@@ -655,6 +722,11 @@ def main():
         return
     if args.summarize_large:
         summarize_large(*args.summarize_large)
+        return
+    if args.summarize_trace:
+        if len(args.summarize_trace) not in (2, 3):
+            sys.exit("--summarize-trace CSV MODE [PATTERN]")
+        summarize_trace(*args.summarize_trace)
         return
     if args.gpus != 1:
         print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "

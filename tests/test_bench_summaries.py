@@ -131,3 +131,40 @@ def test_flux_summary_scales_by_the_copy_kernel(bench, tmp_path):
     out = json.load(open(bench.PROFILE_TRAFFIC))
     assert out["copy_kernel"]["fetch_scale"] == 2.0 and out["copy_kernel"]["write_scale"] == 1.0
     assert out["part_atomic"]["traffic_bytes"] == round((1100.0 * 2.0 + 500.0) * 1024)
+
+
+def write_trace(path, launches):
+    """A kernel-trace CSV: (name, workgroups, start us, end us) per dispatch."""
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Dispatch_Id", "Kernel_Name", "Grid_Size_X", "Workgroup_Size_X", "Start_Timestamp",
+                    "End_Timestamp"])
+        for d, (k, g, a, b) in enumerate(launches, 1):
+            w.writerow([d, k, g * 256, 256, int(a * 1000), int(b * 1000)])
+
+
+def test_trace_bursts_split_on_idle_time(bench, tmp_path, capsys):
+    flux = "void mgcfd::k_flux_part<double, 256, 6, 3, false, 6>(mgcfd::PartInfo const*)"
+    # three back-to-back launches, 100 us idle, then two launches 1 us apart
+    launches = [(flux, 1172, 0, 18), (flux, 1172, 18, 36), (flux, 1172, 36, 54),
+                (flux, 1172, 154, 174), (flux, 1172, 175, 195)]
+    write_trace(tmp_path / "t.csv", launches)
+    bench.summarize_trace(str(tmp_path / "t.csv"), "bursts")
+    rows = [l for l in capsys.readouterr().out.splitlines() if l.startswith("| 0") or l.startswith("| 1")]
+    assert rows == ["| 0 | 3 | 18.00 | 18.00 | 0.00 |", "| 1 | 2 | 20.00 | 21.00 | 1.00 |"]
+
+
+def test_trace_cycle_window_per_cycle_figures(bench, tmp_path, capsys):
+    launches, t = [], 0.0
+    for _ in range(2):                              # two cycles of 18 stage kernels and one k_down
+        for _ in range(18):
+            launches.append((STAGE64, 1172, t, t + 10.0))
+            t += 10.0
+        launches.append(("mgcfd::k_down(int, int const*)", 1172, t, t + 5.0))
+        t += 5.0
+    write_trace(tmp_path / "t.csv", launches[:-1])   # the window ends at the last stage kernel
+    bench.summarize_trace(str(tmp_path / "t.csv"), "cycle")
+    out = capsys.readouterr().out
+    assert "36 fused stage kernels (2 cycles)" in out
+    assert "| `void k_flux_part_stage<double, 256, 6, 3, false, 5>` | 1172 | 18.0 | 10.00 | 180.0 |" in out
+    assert "| `k_down` | 1172 | 0.5 | 5.00 | 2.5 |" in out
