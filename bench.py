@@ -492,7 +492,10 @@ def summarize_trace(stats_or_trace, mode, pattern=None):
     kernel and grid size, per cycle (18 stage kernels per cycle); mode
     'bursts': the dispatches matching 'pattern' in launch order, grouped into
     bursts (a new burst after more than 20 us with none running), with each
-    burst's mean kernel time, start-to-start interval and gap."""
+    burst's mean kernel time, start-to-start interval and gap; mode
+    'counters': a rocprofv3 counter file (run_counter_collection.csv), the
+    mean value per dispatch of each counter for the kernel matching
+    'pattern' (default: the FP64 PART_ATOMIC edge-flux kernel)."""
     def short(n, k=90):
         n = n.replace("mgcfd::", "")
         return n if len(n) <= k else n[:k] + "…"
@@ -542,6 +545,15 @@ def summarize_trace(stats_or_trace, mode, pattern=None):
             s2s = (bu[-1][0] - bu[0][0]) / max(len(bu) - 1, 1) / 1e3
             gap = sum(bu[j + 1][0] - bu[j][1] for j in range(len(bu) - 1)) / max(len(bu) - 1, 1) / 1e3
             out.append(f"| {i} | {len(bu)} | {d:.2f} | {s2s:.2f} | {gap:.2f} |")
+    elif mode == "counters":
+        pat = pattern or VARIANT_KERNELS["part_atomic"]
+        acc = {}
+        for r in rows:
+            if re.search(pat, r["Kernel_Name"]):
+                acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        out.append("| counter | mean per dispatch | dispatches |\n|---|---|---|")
+        for c, v in acc.items():
+            out.append(f"| {c} | {sum(v) / len(v):.4g} | {len(v)} |")
     else:
         sys.exit(f"summarize-trace: unknown mode {mode}")
     print("\n".join(out))
@@ -711,8 +723,9 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/flux_traffic.json from two rocprofv3 counter runs of this script")
     ap.add_argument("--summarize-trace", nargs="+", metavar="ARG",
-                    help="CSV MODE [PATTERN]: markdown tables of a rocprofv3 kernel-statistics file (MODE stats) or "
-                         "kernel trace (MODE cycle or bursts) for the profile summaries")
+                    help="CSV MODE [PATTERN]: markdown tables of a rocprofv3 kernel-statistics file (MODE stats), "
+                         "kernel trace (MODE cycle or bursts) or counter file (MODE counters) for the profile "
+                         "summaries")
     ap.add_argument("--summarize-large", nargs=3, metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/mg_cycle_traffic_8m.json from a kernel-trace run and two counter runs "
                          "of this script's 8M-node solve")

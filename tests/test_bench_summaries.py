@@ -168,3 +168,15 @@ def test_trace_cycle_window_per_cycle_figures(bench, tmp_path, capsys):
     assert "36 fused stage kernels (2 cycles)" in out
     assert "| `void k_flux_part_stage<double, 256, 6, 3, false, 5>` | 1172 | 18.0 | 10.00 | 180.0 |" in out
     assert "| `k_down` | 1172 | 0.5 | 5.00 | 2.5 |" in out
+
+
+def test_counter_means_of_the_matching_kernel(bench, tmp_path, capsys):
+    flux = "void mgcfd::k_flux_part<double, 256, 6, 3, false, 6>(x)"
+    with open(tmp_path / "c.csv", "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writerows([[1, flux, "SQ_WAVES", 4688], [2, flux, "SQ_WAVES", 4688], [1, flux, "SQ_INSTS_VALU", 100],
+                     [2, flux, "SQ_INSTS_VALU", 300], [3, "k_down(int)", "SQ_WAVES", 7]])
+    bench.summarize_trace(str(tmp_path / "c.csv"), "counters")
+    out = capsys.readouterr().out
+    assert "| SQ_WAVES | 4688 | 2 |" in out and "| SQ_INSTS_VALU | 200 | 2 |" in out
