@@ -63,6 +63,7 @@ struct SetImpl {
     std::vector<int> parts;      // part boundaries of the node order (size parts + 1)
     std::vector<double> geo;     // node coordinates [n][3] (original order) when known: geometric order
     bool frozen = false;         // internal order chosen
+    bool priv = false;           // declared by a batch builder, which orders and uploads it itself
     int to_int(int orig) const { return iperm.empty() ? orig : iperm[orig]; }
     int to_orig(int in) const { return perm.empty() ? in : perm[in]; }
 };
@@ -462,10 +463,8 @@ static int build_map_device(op_map m)
     return 0;
 }
 
-// Chooses the internal order of every set declared since the last call, then
-// uploads their maps and pending dats.  Sets, maps and dats may be declared
-// at any time (batched and served solves add meshes after the first loop);
-// an order, once chosen, never changes.
+// f(0) .. f(n - 1) on up to 16 host threads, in chunks of 8 (serially below
+// 64 items).
 template <typename F>
 static void parallel_for(int n, F&& f)
 {
@@ -519,10 +518,21 @@ static int run_tasks(int n, F&& f)
     return 0;
 }
 
-static int freeze()
+// Sets declared by this thread are private to it (SetImpl::priv) while set:
+// a batch builder declares its objects under the registry lock, then orders
+// and uploads them itself without the lock.
+static thread_local bool tl_priv_decl = false;
+
+static bool priv_set(op_set s) { return s && s->impl && S(s)->priv; }
+
+// Orders the given sets that are not frozen yet, then uploads the given maps
+// and dats that are not on the device.  The lists hold either every object
+// of the registry that no builder keeps private (freeze, under the registry
+// lock) or the objects one batch builder declared for itself (without the
+// lock: no other thread touches them).
+static int freeze_objects(const std::vector<op_set>& sets, const std::vector<op_map>& maps,
+                          const std::vector<op_dat>& dats)
 {
-    RtLock lk(g_rt_mu);
-    if (R.frozen && !R.dirty) return 0;
     // node-like sets: targets of dim-2 maps (each set on its own thread)
     struct NodeSet {
         op_set s;
@@ -530,14 +540,17 @@ static int freeze()
         std::vector<int> fsz;
     };
     std::vector<NodeSet> ns;
-    for (auto& s : R.sets) {
+    for (op_set sp : sets) {
+        op_set_core& s = *sp;
         if (!s.impl || S(&s)->frozen) continue;
         NodeSet x{&s, {}, {}};
-        for (auto& m : R.maps)
+        for (op_map mp : maps) {
+            op_map_core& m = *mp;
             if (m.impl && m.to == &s && m.dim == 2 && m.from != &s && !M(&m)->composed) {
                 x.ems.push_back(M(&m));
                 x.fsz.push_back(m.from->size);
             }
+        }
         if (x.ems.empty() || s.size == 0) continue;
         ns.push_back(std::move(x));
     }
@@ -559,15 +572,18 @@ static int freeze()
         return rc;
     // element sets mapped into a renumbered set: order by internal targets
     std::vector<std::pair<op_set, op_map>> es;
-    for (auto& s : R.sets) {
+    for (op_set sp : sets) {
+        op_set_core& s = *sp;
         if (!s.impl) continue;
         SetImpl* si = S(&s);
         if (si->frozen || !si->perm.empty() || s.size == 0) continue;
         op_map key = nullptr;
-        for (auto& m : R.maps)
+        for (op_map mp : maps) {
+            op_map_core& m = *mp;
             if (m.impl && !M(&m)->composed && m.from == &s && m.to->impl && !S(m.to)->perm.empty() &&
                 (key == nullptr || m.dim > key->dim))
                 key = &m;
+        }
         if (key) es.emplace_back(&s, key);
     }
     if (int rc = run_tasks((int)es.size(), [&](int k) -> int {
@@ -638,16 +654,14 @@ static int freeze()
         return 0;
     }))
         return rc;
-    for (auto& s : R.sets)
-        if (s.impl) S(&s)->frozen = true;
-    R.frozen = true;
-    R.dirty = false;
+    for (op_set sp : sets)
+        if (sp->impl) S(sp)->frozen = true;
     // maps and dats permuted into the internal order and uploaded, each on
     // a host thread of its own (HIP calls are thread-safe; each touches only
     // its own object)
     std::vector<op_map> ml;
-    for (auto& m : R.maps)
-        if (m.impl && !M(&m)->on_device) ml.push_back(&m);
+    for (op_map m : maps)
+        if (m->impl && !M(m)->on_device) ml.push_back(m);
     if (int rc = run_tasks((int)ml.size(), [&](int k) -> int {
         if (int r = build_map_device(ml[k])) return r;
         M(ml[k])->on_device = true;
@@ -655,9 +669,33 @@ static int freeze()
     }))
         return rc;
     std::vector<op_dat> dl;
-    for (auto& d : R.dats)
-        if (d.impl && (!D(&d)->d || !D(&d)->pending.empty())) dl.push_back(&d);
+    for (op_dat d : dats)
+        if (d->impl && (!D(d)->d || !D(d)->pending.empty())) dl.push_back(d);
     return run_tasks((int)dl.size(), [&](int k) -> int { return upload_dat(dl[k]); });
+}
+
+// Chooses the internal order of every set declared since the last call, then
+// uploads their maps and pending dats (objects a batch builder keeps private
+// are left to it).  Sets, maps and dats may be declared at any time (batched
+// and served solves add meshes after the first loop); an order, once
+// chosen, never changes.
+static int freeze()
+{
+    RtLock lk(g_rt_mu);
+    if (R.frozen && !R.dirty) return 0;
+    std::vector<op_set> sets;
+    std::vector<op_map> maps;
+    std::vector<op_dat> dats;
+    for (auto& s : R.sets)
+        if (s.impl && !S(&s)->priv) sets.push_back(&s);
+    for (auto& m : R.maps)
+        if (m.impl && !priv_set(m.from) && !priv_set(m.to)) maps.push_back(&m);
+    for (auto& d : R.dats)
+        if (d.impl && !priv_set(d.set)) dats.push_back(&d);
+    if (int rc = freeze_objects(sets, maps, dats)) return rc;
+    R.frozen = true;
+    R.dirty = false;
+    return 0;
 }
 
 // Uploads a dat's pending AoS host data (or zeros) into its device buffer
@@ -1670,6 +1708,7 @@ op_set op_decl_set(int size, const char* name)
     s->size = size;
     s->name = keep_name(name);
     SetImpl* si = new SetImpl();
+    si->priv = tl_priv_decl;
     s->impl = si;
     R.dirty = true;
     return s;
