@@ -1795,6 +1795,8 @@ op_map mgcfd_decl_map_compose(op_map outer, op_map inner, const char* name)
     return add_map(outer->from, inner->to, outer->dim, mi, name);
 }
 
+static op_dat add_dat(op_set set, int dim, const char* type, int size, const char* name, DatImpl* di);
+
 static op_dat decl_dat(op_set set, int dim, const char* type, int size, const char* data, const char* name)
 {
     if (!set || dim < 1 || !type) { fail("op_decl_dat %s: bad arguments", name ? name : ""); return nullptr; }
@@ -1816,13 +1818,36 @@ static op_dat decl_dat(op_set set, int dim, const char* type, int size, const ch
         else
             for (size_t q = 0; q < nel; q++) std::memcpy(di->pending.data() + q * in_esz, data + q * (size_t)size, in_esz);
     }
+    return add_dat(set, dim, type, is_int ? (int)sizeof(int) : size, name, di);
+}
+
+// A dat whose AoS bytes (original element order, 4-byte ints or doubles)
+// the caller hands over: the library's own batched meshes, whose level
+// arrays are built outside the registry lock; no copy.
+static op_dat decl_dat_owned(op_set set, int dim, const char* type, std::vector<char>&& bytes, const char* name)
+{
+    const bool is_int = type && std::strcmp(type, "int") == 0;
+    const size_t esz = is_int ? sizeof(int) : sizeof(double);
+    if (!set || dim < 1 || !type || (!is_int && std::strcmp(type, "double") != 0) ||
+        bytes.size() != esz * dim * (size_t)set->size) {
+        fail("decl_dat %s: bad arguments", name ? name : "");
+        return nullptr;
+    }
+    DatImpl* di = new DatImpl();
+    di->is_int = is_int;
+    di->pending = std::move(bytes);
+    return add_dat(set, dim, type, (int)esz, name, di);
+}
+
+static op_dat add_dat(op_set set, int dim, const char* type, int size, const char* name, DatImpl* di)
+{
     RtLock lk(g_rt_mu);
     R.dats.push_back(op_dat_core{});
     op_dat d = &R.dats.back();
     d->index = (int)R.dats.size() - 1;
     d->set = set;
     d->dim = dim;
-    d->size = is_int ? (int)sizeof(int) : size;
+    d->size = size;
     d->type = keep_name(type);
     d->name = keep_name(name);
     d->impl = di;
