@@ -1134,8 +1134,25 @@ static int build_inverse(op_map m)
     SetImpl* fs = S(m->from);
     SetImpl* ts = S(m->to);
     const int nf = m->from->size, nt = m->to->size, dim = m->dim;
+    // The source elements are split into nch consecutive chunks, each counted
+    // and later placed on a host thread of its own: chunk c's incidences of
+    // target t go after those of chunks < c, so every target keeps its
+    // incidences in element order, as one pass would.  (Counts per chunk and
+    // target: at most 128 MB.)
+    const long long nq = (long long)nf * dim;
+    const int nch = nq < (1 << 18) || nt == 0
+                        ? 1
+                        : (int)std::max(1LL, std::min(8LL, (128LL << 20) / ((long long)sizeof(int) * nt)));
+    std::vector<int> ebound(nch + 1);
+    for (int c = 0; c <= nch; c++) ebound[c] = (int)((long long)nf * c / nch);
+    std::vector<std::vector<int>> ccnt(nch, std::vector<int>(nt, 0));
+    host_tasks(nch, [&](int c) {
+        std::vector<int>& k = ccnt[c];
+        for (size_t q = (size_t)ebound[c] * dim; q < (size_t)ebound[c + 1] * dim; q++) k[ts->to_int(mi->h_orig[q])]++;
+    });
     std::vector<int> cnt(nt, 0);
-    for (size_t q = 0; q < (size_t)nf * dim; q++) cnt[ts->to_int(mi->h_orig[q])]++;
+    for (int c = 0; c < nch; c++)
+        for (int t = 0; t < nt; t++) cnt[t] += ccnt[c][t];
     std::vector<int> tg;
     for (int t = 0; t < nt; t++)
         if (cnt[t]) tg.push_back(t);
@@ -1150,24 +1167,35 @@ static int build_inverse(op_map m)
         P.max_deg = std::max(P.max_deg, ptr[i + 1]);
         ptr[i + 1] += ptr[i];
     }
-    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
     std::vector<int2> inc2(dim == 2 ? ptr[P.n_targets] : 0);
     std::vector<int> inc1(dim == 2 ? 0 : ptr[P.n_targets]);
-    // original element order, column order within an element: OP2 seq order
-    for (int e = 0; e < nf; e++) {
-        const int ein = fs->to_int(e);
-        for (int c = 0; c < dim; c++) {
-            const int t = ts->to_int(mi->h_orig[(size_t)e * dim + c]);
-            const int s = P.dense ? t : slot[t];
-            const int at = fill[s]++;
-            if (dim == 2) {
-                const int other = ts->to_int(mi->h_orig[(size_t)e * dim + (1 - c)]);
-                inc2[at] = make_int2(other, (int)((unsigned)ein | ((unsigned)c << 31)));
-            } else {
-                inc1[at] = ein;
-            }
+    // each chunk's first position in every target's run
+    for (int t = 0; t < nt; t++) {
+        if (!cnt[t]) continue;
+        int at = ptr[P.dense ? t : slot[t]];
+        for (int c = 0; c < nch; c++) {
+            const int k = ccnt[c][t];
+            ccnt[c][t] = at;
+            at += k;
         }
     }
+    // original element order, column order within an element: OP2 seq order
+    host_tasks(nch, [&](int ch) {
+        std::vector<int>& fill = ccnt[ch];
+        for (int e = ebound[ch]; e < ebound[ch + 1]; e++) {
+            const int ein = fs->to_int(e);
+            for (int c = 0; c < dim; c++) {
+                const int t = ts->to_int(mi->h_orig[(size_t)e * dim + c]);
+                const int at = fill[t]++;
+                if (dim == 2) {
+                    const int other = ts->to_int(mi->h_orig[(size_t)e * dim + (1 - c)]);
+                    inc2[at] = make_int2(other, (int)((unsigned)ein | ((unsigned)c << 31)));
+                } else {
+                    inc1[at] = ein;
+                }
+            }
+        }
+    });
     if (!P.dense) {
         HIPCHK(hipMalloc(&P.d_targets, sizeof(int) * std::max<size_t>(1, tg.size())));
         if (!tg.empty()) HIPCHK(hipMemcpy(P.d_targets, tg.data(), sizeof(int) * tg.size(), hipMemcpyHostToDevice));
