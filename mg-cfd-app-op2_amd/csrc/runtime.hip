@@ -662,16 +662,18 @@ static int freeze_objects(const std::vector<op_set>& sets, const std::vector<op_
     std::vector<op_map> ml;
     for (op_map m : maps)
         if (m->impl && !M(m)->on_device) ml.push_back(m);
-    if (int rc = run_tasks((int)ml.size(), [&](int k) -> int {
-        if (int r = build_map_device(ml[k])) return r;
-        M(ml[k])->on_device = true;
-        return 0;
-    }))
-        return rc;
     std::vector<op_dat> dl;
     for (op_dat d : dats)
         if (d->impl && (!D(d)->d || !D(d)->pending.empty())) dl.push_back(d);
-    return run_tasks((int)dl.size(), [&](int k) -> int { return upload_dat(dl[k]); });
+    // maps and dats in one pool: the largest of each (a level's edge map and
+    // edge weights) upload side by side
+    const int nm = (int)ml.size();
+    return run_tasks(nm + (int)dl.size(), [&](int k) -> int {
+        if (k >= nm) return upload_dat(dl[k - nm]);
+        if (int r = build_map_device(ml[k])) return r;
+        M(ml[k])->on_device = true;
+        return 0;
+    });
 }
 
 // Chooses the internal order of every set declared since the last call, then
@@ -713,9 +715,16 @@ static int upload_dat(op_dat d)
     }
     if (di->pending.empty()) return 0;
     std::vector<char> aos(bytes, 0);
-    const size_t rec = esz * dim;
-    for (int i = 0; i < n; i++)
-        std::memcpy(aos.data() + (size_t)i * rec, di->pending.data() + (size_t)si->to_orig(i) * rec, rec);
+    auto permute = [&](auto* dst, const auto* src) {
+        for (int i = 0; i < n; i++) {
+            const size_t o = (size_t)si->to_orig(i) * dim, q = (size_t)i * dim;
+            for (int v = 0; v < dim; v++) dst[q + v] = src[o + v];
+        }
+    };
+    if (di->is_int)
+        permute(reinterpret_cast<int*>(aos.data()), reinterpret_cast<const int*>(di->pending.data()));
+    else
+        permute(reinterpret_cast<double*>(aos.data()), reinterpret_cast<const double*>(di->pending.data()));
     HIPCHK(hipMemcpy(di->d, aos.data(), bytes, hipMemcpyHostToDevice));
     di->pending.clear();
     di->pending.shrink_to_fit();
